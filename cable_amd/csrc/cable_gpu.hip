@@ -1,0 +1,540 @@
+// cable_gpu.hip -- kernels and C ABI of libcable_gpu.so (include/cable_gpu.h).
+//
+// One launch of cbm_kernel performs one timestep for every tile: one tile per
+// thread, 256 threads per workgroup, the per-class parameter tables staged
+// into LDS at the start of each workgroup.  All device memory is owned by the
+// context; HBM traffic per tile-step is the tile's state, parameters, forcing
+// and outputs, read and written once.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cable_gpu.h"
+#include "cbm_step.h"
+
+using namespace cbm;
+
+// ---------------------------------------------------------------------------
+// field table
+namespace {
+enum { ROLE_PARAM = 0, ROLE_FORC = 1, ROLE_STATE = 2, ROLE_DIAG = 3 };
+struct FieldMeta {
+  const char *name;
+  char prec;
+  int n;
+  int role;
+};
+#define PARAM ROLE_PARAM
+#define FORC ROLE_FORC
+#define STATE ROLE_STATE
+#define DIAG ROLE_DIAG
+const FieldMeta kFields[CABLE_NFIELDS] = {
+#define X(id, name, prec, n, role) {name, #prec[0], n, role},
+    CABLE_FIELDS(X)
+#undef X
+};
+#undef PARAM
+#undef FORC
+#undef STATE
+#undef DIAG
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIPCHECK(expr)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(CABLE_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// element size of a field on the device for a precision mode
+size_t dev_elem(const FieldMeta &f, int precision) {
+  switch (f.prec) {
+    case 'I': return 4;
+    case 'D': case 'B': return 8;
+    default: return precision == CABLE_PREC_MIXED ? 4 : 8;
+  }
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// kernels
+
+template <class T>
+__device__ __forceinline__ T ld(const void *p, int64_t i) {
+  return static_cast<const T *>(p)[i];
+}
+template <class T, class V>
+__device__ __forceinline__ void st(void *p, int64_t i, V v) {
+  static_cast<T *>(p)[i] = static_cast<T>(v);
+}
+
+// device type of a field of precision code P for working precision R
+template <class R, char P> struct DevT { using type = R; };
+template <class R> struct DevT<R, 'D'> { using type = double; };
+template <class R> struct DevT<R, 'B'> { using type = double; };
+template <class R> struct DevT<R, 'I'> { using type = int; };
+
+constexpr int role_id(int r) { return r; }
+
+template <class R, bool FULL, bool TABLES>
+__global__ void __launch_bounds__(256) cbm_kernel(StepArgs a) {
+  __shared__ double lds_v[TABLES ? NVEGCLASS * VC_N : 1];
+  __shared__ double lds_s[TABLES ? NSOILCLASS * SC_N : 1];
+  if constexpr (TABLES) {
+    for (int i = threadIdx.x; i < NVEGCLASS * VC_N; i += blockDim.x) lds_v[i] = a.vtab[i];
+    for (int i = threadIdx.x; i < NSOILCLASS * SC_N; i += blockDim.x) lds_s[i] = a.stab[i];
+    __syncthreads();
+  }
+  const int64_t mp = a.mp;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int passes = 0;
+  if (t < mp) {
+    void *const *F = a.F;
+    Tile<R> T;
+    // ---- gather ----
+    if constexpr (FULL) {
+#define PARAM ROLE_PARAM
+#define FORC ROLE_FORC
+#define STATE ROLE_STATE
+#define DIAG ROLE_DIAG
+#define X(id, name, prec, n, role)                                                    \
+  {                                                                                   \
+    using DT = typename DevT<R, #prec[0]>::type;                                      \
+    const int64_t off = (role == ROLE_FORC) ? a.foff : 0;                             \
+    _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] =                           \
+        (decltype(T.id[0] + 0))ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t + off);     \
+  }
+      CABLE_FIELDS(X)
+#undef X
+#undef PARAM
+#undef FORC
+#undef STATE
+#undef DIAG
+    } else {
+      // prognostic state
+#define LDS_(id, n)                                                                   \
+  {                                                                                   \
+    using DT = typename DevT<R, 'F'>::type;                                           \
+    _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t); \
+  }
+#define LDD_(id, n)                                                                   \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<double>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+      T.ssnow_isflag[0] = ld<int>(F[CABLE_F_ssnow_isflag], t);
+      LDS_(ssnow_pudsto, 1) LDS_(ssnow_rtsoil, 1) LDS_(ssnow_wbtot1, 1) LDS_(ssnow_wbtot2, 1)
+      LDS_(ssnow_wb_lake, 1) LDS_(ssnow_owetfac, 1) LDS_(ssnow_snage, 1) LDS_(ssnow_snowd, 1)
+      LDS_(ssnow_ssdnn, 1) LDS_(ssnow_tss, 1) LDS_(ssnow_smass, 3) LDS_(ssnow_ssdn, 3)
+      LDS_(ssnow_tgg, 6) LDS_(ssnow_tggsn, 3) LDS_(canopy_cansto, 1) LDS_(bgc_cplant, 3)
+      LDS_(bgc_csoil, 2)
+      LDD_(ssnow_wb, 6) LDD_(ssnow_wbice, 6) LDD_(drv_owb, 1)
+      LDD_(bal_ebal_tot, 1) LDD_(bal_wbal_tot, 1) LDD_(bal_precip_tot, 1) LDD_(bal_rnoff_tot, 1)
+      LDD_(bal_evap_tot, 1) LDD_(bal_radbalsum, 1)
+      T.ssnow_gammzz[0] = ld<double>(F[CABLE_F_ssnow_gammzz], t);
+      // soil liquid water as the previous step left it (wb - wbice), before
+      // this step's lake fix-up
+#pragma unroll
+      for (int k = 0; k < MS; k++) T.ssnow_wbliq[k] = T.ssnow_wb[k] - T.ssnow_wbice[k];
+      // per-tile parameters
+      T.veg_iveg[0] = ld<int>(F[CABLE_F_veg_iveg], t);
+      T.soil_isoilm[0] = ld<int>(F[CABLE_F_soil_isoilm], t);
+      LDS_(veg_vlai, 1)
+      T.soil_albsoil[0] = ld<typename DevT<R, 'F'>::type>(F[CABLE_F_soil_albsoil], t);
+      // compact forcing (offline reader's derivations)
+      {
+        using DT = typename DevT<R, 'F'>::type;
+        const int64_t i = t + a.foff;
+        R fsd = ld<DT>(F[CABLE_F_met_fsd], i);
+        T.met_fsd[0] = fsd;
+        T.met_fsd[1] = fsd;
+        R tk = ld<DT>(F[CABLE_F_met_tk], i);
+        T.met_tk[0] = tk;
+        T.met_tvair[0] = tk;
+        T.met_tvrad[0] = tk;
+        T.met_fld[0] = ld<DT>(F[CABLE_F_met_fld], i);
+        T.met_qv[0] = ld<DT>(F[CABLE_F_met_qv], i);
+        T.met_ua[0] = ld<DT>(F[CABLE_F_met_ua], i);
+        T.met_pmb[0] = ld<DT>(F[CABLE_F_met_pmb], i);
+        R pr = ld<DT>(F[CABLE_F_met_precip], i);
+        T.met_precip[0] = pr;
+        T.met_precip_sn[0] = (tk <= R(273.16)) ? pr : R(0.0);
+        T.met_coszen[0] = ld<DT>(F[CABLE_F_met_coszen], i);
+        T.met_ca[0] = (R)a.ca;
+        T.met_doy[0] = (R)a.doy;
+      }
+#undef LDS_
+#undef LDD_
+    }
+    const int iv = T.veg_iveg[0] - 1, is = T.soil_isoilm[0] - 1;
+    const double *vrow = TABLES ? lds_v + (iv < 0 ? 0 : iv) * VC_N : nullptr;
+    const double *srow = TABLES ? lds_s + (is < 0 ? 0 : is) * SC_N : nullptr;
+    Stepper<R, FULL, TABLES> S(T, a, vrow, srow);
+    S.run();
+    passes = S.npass;
+    // ---- scatter ----
+    if constexpr (FULL) {
+#define PARAM ROLE_PARAM
+#define FORC ROLE_FORC
+#define STATE ROLE_STATE
+#define DIAG ROLE_DIAG
+#define X(id, name, prec, n, role)                                                      \
+  if (role == ROLE_STATE || role == ROLE_DIAG || CABLE_F_##id == CABLE_F_met_tvair ||   \
+      CABLE_F_##id == CABLE_F_veg_vlai) {                                               \
+    using DT = typename DevT<R, #prec[0]>::type;                                        \
+    const int64_t off = (role == ROLE_FORC) ? a.foff : 0;                               \
+    _Pragma("unroll") for (int k = 0; k < n; k++)                                       \
+        st<DT>(F[CABLE_F_##id], (int64_t)k * mp + t + off, T.id[k]);                    \
+  }
+      CABLE_FIELDS(X)
+#undef X
+#undef PARAM
+#undef FORC
+#undef STATE
+#undef DIAG
+    } else {
+      using DT = typename DevT<R, 'F'>::type;
+#define STS_(id, n) \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) st<DT>(F[CABLE_F_##id], (int64_t)k * mp + t, T.id[k]); }
+#define STD_(id, n) \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) st<double>(F[CABLE_F_##id], (int64_t)k * mp + t, T.id[k]); }
+      st<int>(F[CABLE_F_ssnow_isflag], t, T.ssnow_isflag[0]);
+      STS_(ssnow_pudsto, 1) STS_(ssnow_rtsoil, 1) STS_(ssnow_wbtot1, 1) STS_(ssnow_wbtot2, 1)
+      STS_(ssnow_wb_lake, 1) STS_(ssnow_owetfac, 1) STS_(ssnow_snage, 1) STS_(ssnow_snowd, 1)
+      STS_(ssnow_ssdnn, 1) STS_(ssnow_tss, 1) STS_(ssnow_smass, 3) STS_(ssnow_ssdn, 3)
+      STS_(ssnow_tgg, 6) STS_(ssnow_tggsn, 3) STS_(canopy_cansto, 1) STS_(bgc_cplant, 3)
+      STS_(bgc_csoil, 2)
+      STD_(ssnow_wb, 6) STD_(ssnow_wbice, 6) STD_(drv_owb, 1)
+      STD_(bal_ebal_tot, 1) STD_(bal_wbal_tot, 1) STD_(bal_precip_tot, 1) STD_(bal_rnoff_tot, 1)
+      STD_(bal_evap_tot, 1) STD_(bal_radbalsum, 1)
+      st<double>(F[CABLE_F_ssnow_gammzz], t, T.ssnow_gammzz[0]);
+      // the eight headline fluxes
+      STS_(canopy_rnet, 1) STS_(canopy_fe, 1) STS_(canopy_fh, 1) STS_(canopy_fev, 1)
+      STD_(canopy_fes, 1) STS_(canopy_fgpp, 1) STS_(ssnow_runoff, 1) STS_(rad_trad, 1)
+#undef STS_
+#undef STD_
+    }
+  }
+  if (a.npass) {
+    // wave-level sum, one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) passes += __shfl_xor(passes, o, 64);
+    if ((threadIdx.x & 63) == 0 && passes) atomicAdd(a.npass, passes);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// context
+
+struct cable_gpu_ctx {
+  int64_t mp = 0;
+  int precision = 0;
+  std::vector<void *> dev;          // per field device arrays (n*mp elements)
+  void **d_tab = nullptr;           // device copy of `dev`
+  void **d_tab_run = nullptr;       // `dev` with forcing ids -> resident forcing
+  std::vector<void *> forc;         // resident forcing arrays (8)
+  int nsteps_forc = 0;
+  double ca = 400e-6, doy = 1.0;
+  double zse[6] = {0}, zshh[7] = {0}, ratecp[3] = {0}, ratecs[2] = {0};
+  double *d_vtab = nullptr, *d_stab = nullptr;
+  bool have_tables = false;
+  bool first_call = true;
+  int *d_npass = nullptr;
+  int64_t last_passes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int device = 0;
+};
+
+namespace {
+size_t field_bytes(const cable_gpu_ctx *c, int id) {
+  return dev_elem(kFields[id], c->precision) * (size_t)kFields[id].n * (size_t)c->mp;
+}
+
+StepArgs make_args(const cable_gpu_ctx *c, void **tab, int ktau, double dels, int64_t foff) {
+  StepArgs a;
+  memset(&a, 0, sizeof a);
+  a.F = tab;
+  a.mp = c->mp;
+  a.foff = foff;
+  a.ktau = ktau;
+  a.first_call = c->first_call ? 1 : 0;
+  a.dels = dels;
+  memcpy(a.zse, c->zse, sizeof a.zse);
+  memcpy(a.zshh, c->zshh, sizeof a.zshh);
+  memcpy(a.ratecp, c->ratecp, sizeof a.ratecp);
+  memcpy(a.ratecs, c->ratecs, sizeof a.ratecs);
+  a.vtab = c->d_vtab;
+  a.stab = c->d_stab;
+  a.ca = c->ca;
+  a.doy = c->doy;
+  a.npass = c->d_npass;
+  return a;
+}
+
+template <class R, bool FULL, bool TABLES>
+hipError_t launch(const cable_gpu_ctx *c, const StepArgs &a) {
+  const int block = 256;
+  const int64_t grid = (c->mp + block - 1) / block;
+  hipLaunchKernelGGL((cbm_kernel<R, FULL, TABLES>), dim3((unsigned)grid), dim3(block), 0, c->stream, a);
+  return hipGetLastError();
+}
+
+// host conversion between a host array and the device element type
+int copy_to_device(cable_gpu_ctx *c, void *dst, const FieldMeta &f, const void *host, int host_type,
+                   size_t count) {
+  size_t de = dev_elem(f, c->precision);
+  std::vector<unsigned char> buf(de * count);
+  for (size_t i = 0; i < count; i++) {
+    double v;
+    switch (host_type) {
+      case CABLE_HOST_F64: v = ((const double *)host)[i]; break;
+      case CABLE_HOST_F32: v = ((const float *)host)[i]; break;
+      case CABLE_HOST_I32: v = ((const int32_t *)host)[i]; break;
+      default: return fail(CABLE_E_ARG, "bad host type %d", host_type);
+    }
+    if (f.prec == 'I')
+      ((int32_t *)buf.data())[i] = (int32_t)llround(v);
+    else if (de == 8)
+      ((double *)buf.data())[i] = v;
+    else
+      ((float *)buf.data())[i] = (float)v;
+  }
+  HIPCHECK(hipMemcpyAsync(dst, buf.data(), buf.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return CABLE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cable_gpu_nfields(void) { return CABLE_NFIELDS; }
+
+int cable_gpu_field_info(int id, const char **name, char *prec, int *n, int *role) {
+  if (id < 0 || id >= CABLE_NFIELDS) return fail(CABLE_E_ARG, "field id %d out of range", id);
+  if (name) *name = kFields[id].name;
+  if (prec) *prec = kFields[id].prec;
+  if (n) *n = kFields[id].n;
+  if (role) *role = kFields[id].role;
+  return CABLE_OK;
+}
+
+int cable_gpu_field_id(const char *name) {
+  if (!name) return -1;
+  for (int i = 0; i < CABLE_NFIELDS; i++)
+    if (strcmp(kFields[i].name, name) == 0) return i;
+  return -1;
+}
+
+const char *cable_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int cable_gpu_create(cable_gpu_ctx **out, int64_t mp, int precision) {
+  if (!out || mp <= 0) return fail(CABLE_E_ARG, "cable_gpu_create: bad arguments (mp=%lld)", (long long)mp);
+  if (precision != CABLE_PREC_FP64 && precision != CABLE_PREC_MIXED)
+    return fail(CABLE_E_ARG, "cable_gpu_create: unknown precision %d", precision);
+  cable_gpu_ctx *c = new cable_gpu_ctx();
+  c->mp = mp;
+  c->precision = precision;
+  c->dev.assign(CABLE_NFIELDS, nullptr);
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  for (int i = 0; i < CABLE_NFIELDS && e == hipSuccess; i++) {
+    e = hipMalloc(&c->dev[i], field_bytes(c, i));
+    if (e == hipSuccess) e = hipMemsetAsync(c->dev[i], 0, field_bytes(c, i), c->stream);
+  }
+  if (e == hipSuccess) e = hipMalloc(&c->d_tab, sizeof(void *) * CABLE_NFIELDS);
+  if (e == hipSuccess) e = hipMalloc(&c->d_tab_run, sizeof(void *) * CABLE_NFIELDS);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->d_tab, c->dev.data(), sizeof(void *) * CABLE_NFIELDS, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMalloc(&c->d_npass, sizeof(int));
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    cable_gpu_destroy(c);
+    return fail(CABLE_E_HIP, "cable_gpu_create: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return CABLE_OK;
+}
+
+int cable_gpu_destroy(cable_gpu_ctx *c) {
+  if (!c) return CABLE_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void *p : c->dev)
+    if (p) hipFree(p);
+  for (void *p : c->forc)
+    if (p) hipFree(p);
+  if (c->d_tab) hipFree(c->d_tab);
+  if (c->d_tab_run) hipFree(c->d_tab_run);
+  if (c->d_vtab) hipFree(c->d_vtab);
+  if (c->d_stab) hipFree(c->d_stab);
+  if (c->d_npass) hipFree(c->d_npass);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return CABLE_OK;
+}
+
+int cable_gpu_upload(cable_gpu_ctx *c, int id, const void *host, int host_type) {
+  if (!c || !host || id < 0 || id >= CABLE_NFIELDS) return fail(CABLE_E_ARG, "cable_gpu_upload: bad arguments");
+  return copy_to_device(c, c->dev[id], kFields[id], host, host_type, (size_t)kFields[id].n * c->mp);
+}
+
+int cable_gpu_download(cable_gpu_ctx *c, int id, void *host, int host_type) {
+  if (!c || !host || id < 0 || id >= CABLE_NFIELDS) return fail(CABLE_E_ARG, "cable_gpu_download: bad arguments");
+  const FieldMeta &f = kFields[id];
+  size_t count = (size_t)f.n * c->mp, de = dev_elem(f, c->precision);
+  std::vector<unsigned char> buf(de * count);
+  HIPCHECK(hipMemcpyAsync(buf.data(), c->dev[id], buf.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < count; i++) {
+    double v = f.prec == 'I' ? (double)((int32_t *)buf.data())[i]
+               : de == 8     ? ((double *)buf.data())[i]
+                             : (double)((float *)buf.data())[i];
+    switch (host_type) {
+      case CABLE_HOST_F64: ((double *)host)[i] = v; break;
+      case CABLE_HOST_F32: ((float *)host)[i] = (float)v; break;
+      case CABLE_HOST_I32: ((int32_t *)host)[i] = (int32_t)llround(v); break;
+      default: return fail(CABLE_E_ARG, "bad host type %d", host_type);
+    }
+  }
+  return CABLE_OK;
+}
+
+int cable_gpu_set_globals(cable_gpu_ctx *c, const double *zse, const double *zshh, const double *ratecp,
+                          const double *ratecs) {
+  if (!c || !zse || !zshh || !ratecp || !ratecs) return fail(CABLE_E_ARG, "cable_gpu_set_globals: null argument");
+  memcpy(c->zse, zse, sizeof c->zse);
+  memcpy(c->zshh, zshh, sizeof c->zshh);
+  memcpy(c->ratecp, ratecp, sizeof c->ratecp);
+  memcpy(c->ratecs, ratecs, sizeof c->ratecs);
+  return CABLE_OK;
+}
+
+int cable_gpu_step(cable_gpu_ctx *c, int ktau, double dels) {
+  if (!c) return fail(CABLE_E_ARG, "cable_gpu_step: null context");
+  if (ktau < 1 || !(dels > 0)) return fail(CABLE_E_ARG, "cable_gpu_step: bad ktau %d or dels %g", ktau, dels);
+  HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(int), c->stream));
+  StepArgs a = make_args(c, c->d_tab, ktau, dels, 0);
+  hipError_t e = c->precision == CABLE_PREC_MIXED ? launch<float, true, false>(c, a)
+                                                   : launch<double, true, false>(c, a);
+  if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_step launch: %s", hipGetErrorString(e));
+  int np = 0;
+  HIPCHECK(hipMemcpyAsync(&np, c->d_npass, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->last_passes = np;
+  c->first_call = false;
+  return CABLE_OK;
+}
+
+int cable_gpu_veg_cols(void) { return VC_N; }
+int cable_gpu_soil_cols(void) { return SC_N; }
+int cable_gpu_table_column(int table, int id) {
+  if (id < 0 || id >= CABLE_NFIELDS) return -1;
+  switch (id) {
+#define X(fid, name, prec, n, role) \
+  case CABLE_F_##fid: return table == 0 ? veg_col(CABLE_F_##fid) : soil_col(CABLE_F_##fid);
+    CABLE_FIELDS(X)
+#undef X
+  }
+  return -1;
+}
+
+int cable_gpu_set_class_tables(cable_gpu_ctx *c, const double *veg, const double *soil) {
+  if (!c || !veg || !soil) return fail(CABLE_E_ARG, "cable_gpu_set_class_tables: null argument");
+  if (!c->d_vtab) HIPCHECK(hipMalloc(&c->d_vtab, sizeof(double) * NVEGCLASS * VC_N));
+  if (!c->d_stab) HIPCHECK(hipMalloc(&c->d_stab, sizeof(double) * NSOILCLASS * SC_N));
+  HIPCHECK(hipMemcpyAsync(c->d_vtab, veg, sizeof(double) * NVEGCLASS * VC_N, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->d_stab, soil, sizeof(double) * NSOILCLASS * SC_N, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->have_tables = true;
+  return CABLE_OK;
+}
+
+int cable_gpu_set_forcing(cable_gpu_ctx *c, int nsteps, const void *fsd1, const void *fld, const void *tk,
+                          const void *qv, const void *ua, const void *pmb, const void *precip, const void *coszen,
+                          int host_type, double ca, double doy) {
+  if (!c || nsteps <= 0) return fail(CABLE_E_ARG, "cable_gpu_set_forcing: bad arguments");
+  const void *src[8] = {fsd1, fld, tk, qv, ua, pmb, precip, coszen};
+  const int ids[8] = {CABLE_F_met_fsd, CABLE_F_met_fld, CABLE_F_met_tk, CABLE_F_met_qv,
+                      CABLE_F_met_ua, CABLE_F_met_pmb, CABLE_F_met_precip, CABLE_F_met_coszen};
+  for (int i = 0; i < 8; i++)
+    if (!src[i]) return fail(CABLE_E_ARG, "cable_gpu_set_forcing: null forcing array %d", i);
+  for (void *p : c->forc)
+    if (p) hipFree(p);
+  c->forc.assign(8, nullptr);
+  FieldMeta fm = {"forcing", 'F', 1, ROLE_FORC};
+  size_t count = (size_t)nsteps * c->mp;
+  size_t bytes = dev_elem(fm, c->precision) * count;
+  std::vector<void *> tab = c->dev;
+  for (int i = 0; i < 8; i++) {
+    HIPCHECK(hipMalloc(&c->forc[i], bytes));
+    int r = copy_to_device(c, c->forc[i], fm, src[i], host_type, count);
+    if (r) return r;
+    tab[ids[i]] = c->forc[i];
+  }
+  HIPCHECK(hipMemcpyAsync(c->d_tab_run, tab.data(), sizeof(void *) * CABLE_NFIELDS, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->nsteps_forc = nsteps;
+  c->ca = ca;
+  c->doy = doy;
+  return CABLE_OK;
+}
+
+int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
+  if (!c) return fail(CABLE_E_ARG, "cable_gpu_run: null context");
+  if (!c->have_tables || c->nsteps_forc == 0)
+    return fail(CABLE_E_STATE, "cable_gpu_run: class tables and forcing must be set first");
+  if (k0 < 0 || nsteps <= 0 || k0 + nsteps > c->nsteps_forc || !(dels > 0))
+    return fail(CABLE_E_ARG, "cable_gpu_run: steps %d..%d outside the %d resident forcing steps", k0,
+                k0 + nsteps - 1, c->nsteps_forc);
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  for (int k = k0; k < k0 + nsteps; k++) {
+    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
+    a.npass = nullptr;
+    hipError_t e = c->precision == CABLE_PREC_MIXED ? launch<float, false, true>(c, a)
+                                                     : launch<double, false, true>(c, a);
+    if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run launch: %s", hipGetErrorString(e));
+    c->first_call = false;
+  }
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  HIPCHECK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  return CABLE_OK;
+}
+
+int cable_gpu_sync(cable_gpu_ctx *c) {
+  if (!c) return fail(CABLE_E_ARG, "cable_gpu_sync: null context");
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return CABLE_OK;
+}
+
+int cable_gpu_last_run_ms(cable_gpu_ctx *c, double *ms) {
+  if (!c || !ms) return fail(CABLE_E_ARG, "cable_gpu_last_run_ms: null argument");
+  *ms = c->last_ms;
+  return CABLE_OK;
+}
+
+int cable_gpu_counters(cable_gpu_ctx *c, int64_t *passes) {
+  if (!c || !passes) return fail(CABLE_E_ARG, "cable_gpu_counters: null argument");
+  *passes = c->last_passes;
+  return CABLE_OK;
+}
+
+}  // extern "C"

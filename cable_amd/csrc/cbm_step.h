@@ -1,0 +1,1982 @@
+// cbm_step.h -- the per-tile CABLE timestep on the GPU (device code).
+//
+// Stepper<R, FULL, TABLES>::run() performs, for the tile held in `t`, one
+// call of the reference's cbm() together with the offline driver's contract
+// around it; see cbm_physics.h for the template parameters and layout.
+#pragma once
+#include "cbm_physics.h"
+
+namespace cbm {
+
+template <class R, bool FULL, bool TABLES>
+struct Stepper {
+  Tile<R> &t;
+  const StepArgs &a;
+  const double *vrow, *srow;
+  int npass = 0;
+
+  // ---- constants: src/params/*, src/util/cable_common.F90:217-222 ----
+  static constexpr R TFRZ = R(273.16), SBOLTZ = R(5.67e-8), EMSOIL = R(1.0),
+                     EMLEAF = R(1.0), CAPP = R(1004.64), HL = R(2.5014e6),
+                     HLF = R(0.334e6), DHEAT = R(21.5e-6), GRAV = R(9.8086),
+                     RGAS = R(8.3143), RMAIR = R(0.02897), RMH2O = R(0.018016),
+                     CGSNOW = R(2090.0), CSICE = R(2.100e3), CSWAT = R(4.218e3),
+                     DENSITY_LIQ = R(1000.0), DENSITY_ICE = R(921.0),
+                     TETENA = R(6.106), TETENB = R(17.27), TETENC = R(237.3),
+                     VONK = R(0.40), A33 = R(1.25), CSW = R(0.50), CTL = R(0.40),
+                     APOL = R(0.70), PRANDT = R(0.71), CRD = R(0.3), CSD = R(0.003),
+                     CCD = R(15.0), CCW_C = R(2.0), USUHM = R(0.3), ZETA0 = R(0.0),
+                     ZETNEG = R(-15.0), ZETPOS = R(1.0), ZDLIN = R(1.0), UMIN = R(0.1),
+                     PI_C = R(3.1415927), LAI_THRESH = R(0.001), RAD_THRESH = R(0.001),
+                     COSZEN_TOLS = R(1.0e-4), GAM0 = R(28.0e-6), GAM1 = R(0.0509),
+                     GAM2 = R(0.0010), RGBWC = R(1.32), RGSWC = R(1.57),
+                     TREFK = R(298.2), MAX_GLACIER_SNOWD = R(1100.0),
+                     SNOW_CCNSW = R(2.0), MAX_SSDN = R(750.0), MAX_SCONDS = R(2.51),
+                     FROZEN_LIMIT = R(0.85), SNMIN = R(1.0);
+  static constexpr int LAKES = 16, ICE_VEG = 17, ICE_SOIL = 9;
+
+  CBM_HD Stepper(Tile<R> &t_, const StepArgs &a_, const double *v, const double *s)
+      : t(t_), a(a_), vrow(v), srow(s) {}
+
+  // ===================================================================
+  // src/science/roughness/roughnessHGT_effLAI_cbl.F90:42-149,
+  // src/science/roughness/cable_roughness.F90:64-317
+  CBM_HD void ruff_resist() {
+    const R z0soilsn_min = K(1.e-7), z0soilsn_min_PF = K(1.e-4);
+    R sde = MAX(K(100.0), t.ssnow_ssdnn[0]);
+    R hmin = K(10.0) * z0soilsn_min;
+    R hc = PV(veg_hc, 0);
+    R hcomp = hc - (K(1.2) * t.ssnow_snowd[0] / sde);
+    R hruff = MAX(hmin, hcomp);
+    t.rough_hruff[0] = hruff;
+    R frac = hruff / MAX(K(0.01), hc);
+    R vlaiw = t.veg_vlai[0] * frac;
+    t.canopy_vlaiw[0] = vlaiw;
+    t.canopy_rghlai[0] = vlaiw;
+    R z0soil = K(0.0009) * MIN(K(1.0), vlaiw) + K(1.e-4);
+    R z0soilsn = z0soil;
+    R snowd = t.ssnow_snowd[0];
+    if (snowd > K(0.01))
+      z0soilsn = MAX(z0soilsn_min, z0soil - z0soil * MIN(snowd, K(10.)) / K(10.));
+    if (snowd > K(0.01) && t.veg_iveg[0] == ICE_VEG) z0soilsn = MAX(z0soilsn, z0soilsn_min_PF);
+    t.rough_z0soil[0] = z0soil;
+    t.rough_z0soilsn[0] = z0soilsn;
+    R rghlai = vlaiw;
+    R za_uv = PV(rough_za_uv, 0), za_tq = PV(rough_za_tq, 0);
+    if (vlaiw <= LAI_THRESH || hruff < z0soilsn) {
+      R z0m = z0soilsn;
+      t.rough_z0m[0] = z0m;
+      t.rough_rt0us[0] = K(0.0);
+      t.rough_disp[0] = K(0.0);
+      R zuv = MAX(K(3.5) + z0m, za_uv);
+      R ztq = MAX(K(3.5) + z0m, za_tq);
+      zuv = MAX(zuv, hruff - K(0.0));
+      ztq = MAX(ztq, hruff - K(0.0));
+      t.rough_zref_uv[0] = zuv;
+      t.rough_zref_tq[0] = ztq;
+      t.rough_zruffs[0] = K(0.0);
+      t.rough_rt1usa[0] = K(0.0);
+      t.rough_rt1usb[0] = K(0.0);
+      R usuh = MIN(sqrt(CSD + CRD * (vlaiw * K(0.5))), USUHM);
+      R xx = sqrt(CCD * MAX((vlaiw * K(0.5)), K(0.0005)));
+      R dh = K(1.0) - (K(1.0) - exp(-xx)) / xx;
+      t.rough_usuh[0] = usuh;
+      t.rough_coexp[0] = usuh / (VONK * CCW_C * (K(1.0) - dh));
+    } else {
+      R usuh = MIN(sqrt(CSD + CRD * (rghlai * K(0.5))), USUHM);
+      R xx = sqrt(CCD * MAX((rghlai * K(0.5)), K(0.0005)));
+      R dh = K(1.0) - (K(1.0) - exp(-xx)) / xx;
+      R disp = dh * hruff;
+      R z0m = ((K(1.0) - dh) * exp(log(CCW_C) - K(1.) + K(1.) / CCW_C - VONK / usuh)) * hruff;
+      R zuv = MAX(K(3.5) + z0m, za_uv);
+      R ztq = MAX(K(3.5) + z0m, za_tq);
+      zuv = MAX(zuv, hruff - disp);
+      ztq = MAX(ztq, hruff - disp);
+      R coexp = usuh / (VONK * CCW_C * (K(1.0) - dh));
+      R term2 = exp(K(2) * CSW * rghlai * (K(1) - disp / hruff));
+      R term3 = A33 * A33 * CTL * K(2) * CSW * rghlai;
+      R term5 = MAX((K(2.) / K(3.)) * hruff / disp, K(1.0));
+      R term6 = exp(K(3.) * coexp * (disp / hruff - K(1.)));
+      R term6a = exp(coexp * (K(0.1) * hruff / hruff - K(1.)));
+      R rt0us = term5 * (ZDLIN * log(ZDLIN * disp / z0soilsn) + (K(1) - ZDLIN)) *
+                (exp(K(2) * CSW * rghlai) - term2) / term3;
+      R zruffs = disp + hruff * (A33 * A33) * CTL / VONK / term5;
+      R rt1usa = term5 * (term2 - K(1.0)) / term3;
+      R rt1usb = term5 * (MIN(ztq + disp, zruffs) - hruff) / (A33 * A33 * CTL * hruff);
+      rt1usb = MAX(rt1usb, K(0.0));
+      t.rough_usuh[0] = usuh;
+      t.rough_disp[0] = disp;
+      t.rough_z0m[0] = z0m;
+      t.rough_zref_uv[0] = zuv;
+      t.rough_zref_tq[0] = ztq;
+      t.rough_coexp[0] = coexp;
+      t.rough_term2[0] = term2;
+      t.rough_term3[0] = term3;
+      t.rough_term5[0] = term5;
+      t.rough_term6[0] = term6;
+      t.rough_term6a[0] = term6a;
+      t.rough_rt0us[0] = rt0us;
+      t.rough_zruffs[0] = zruffs;
+      t.rough_rt1usa[0] = rt1usa;
+      t.rough_rt1usb[0] = rt1usb;
+    }
+  }
+
+  // src/science/misc/cable_air.F90:51-97
+  CBM_HD void define_air() {
+    R tv = t.met_tvair[0], pmb = t.met_pmb[0];
+    R es = TETENA * exp(TETENB * (tv - TFRZ) / (TETENC + (tv - TFRZ)));
+    R cmolar = pmb * K(100.0) / (RGAS * (tv));
+    t.air_cmolar[0] = cmolar;
+    t.air_rho[0] = MIN(K(1.3), RMAIR * cmolar);
+    t.air_volm[0] = RGAS * (tv) / (K(100.0) * pmb);
+    R rlam = HL;
+    t.air_rlam[0] = rlam;
+    t.air_qsat[0] = (RMH2O / RMAIR) * es / pmb;
+    t.air_epsi[0] = (rlam / CAPP) * (RMH2O / RMAIR) * es * TETENB * TETENC /
+                    p2(TETENC + (tv - TFRZ)) / pmb;
+    t.air_visc[0] = K(1e-5) * MAX(K(1.0), K(1.35) + K(0.0092) * (tv - TFRZ));
+    t.air_psyc[0] = pmb * K(100.0) * CAPP * RMAIR / rlam / RMH2O;
+    t.air_dsatdk[0] = K(100.0) * (TETENA * TETENB * TETENC) / p2((tv - TFRZ) + TETENC) *
+                      exp(TETENB * (tv - TFRZ) / ((tv - TFRZ) + TETENC));
+  }
+
+  // src/science/canopy/cbl_qsat.F90:16-53
+  CBM_HD static R qsatfjh(R tair, R pmb) {
+    return (RMH2O / RMAIR) * (TETENA * exp(TETENB * tair / (TETENC + tair))) / pmb;
+  }
+
+  // src/science/radiation/cbl_spitter.F90:36-76
+  CBM_HD static R spitter(int doy, R coszen, R fsd) {
+    const R solcon = K(1370.0);
+    R fbeam = K(0.0), tmprat;
+    R tmpr = K(0.847) + coszen * (K(1.04) * coszen - K(1.61));
+    R tmpk = (K(1.47) - tmpr) / K(1.66);
+    if (coszen > K(1.0e-10) && fsd > K(10.0))
+      tmprat = fsd / (solcon * (K(1.0) + K(0.033) * cos(K(2.0) * PI_C * ((R)doy - K(10.0)) /
+                                                        K(365.0))) * coszen);
+    else
+      tmprat = K(0.0);
+    if (tmprat > K(0.22)) fbeam = K(6.4) * p2(tmprat - K(0.22));
+    if (tmprat > K(0.35)) fbeam = MIN(K(1.66) * tmprat - K(0.4728), K(1.0));
+    if (tmprat > tmpk) fbeam = MAX(K(1.0) - tmpr, K(0.0));
+    return fbeam;
+  }
+
+  // src/science/radiation/cbl_init_radiation.F90:30-392, cbl_rhoch.F90:38-62,
+  // src/science/albedo/cbl_albedo.F90:56-386, cbl_snow_albedo.F90:36-161
+  CBM_HD void init_radiation_albedo(bool veg_mask) {
+    const R pi180 = PI_C / K(180.0);
+    const R GW0 = K(0.308), GW1 = K(0.514), GW2 = K(0.178);
+    R vlaiw = t.canopy_vlaiw[0], coszen = t.met_coszen[0];
+    R cos3[3] = {cos(pi180 * K(15.0)), cos(pi180 * K(45.0)), cos(pi180 * K(75.0))};
+    R xphi1 = K(0.0), xphi2 = K(0.0);
+    if (veg_mask) {
+      R xf = PV(veg_xfang, 0);
+      xphi1 = K(0.5) - xf * (K(0.633) + K(0.33) * xf);
+      xphi2 = K(0.877) * (K(1.0) - K(2.0) * xphi1);
+    }
+    R xk[3], c1[3], rhoch[3];
+#pragma unroll
+    for (int b = 0; b < 3; b++) xk[b] = (vlaiw > LAI_THRESH) ? xphi1 / cos3[b] + xphi2 : K(0.0);
+    c1[0] = sqrt(K(1.0) - PV(veg_taul, 0) - PV(veg_refl, 0));
+    c1[1] = sqrt(K(1.0) - PV(veg_taul, 1) - PV(veg_refl, 1));
+    c1[2] = K(1.0);
+#pragma unroll
+    for (int b = 0; b < 3; b++) rhoch[b] = (K(1.0) - c1[b]) / (K(1.0) + c1[b]);
+    R tiny = COSZEN_TOLS * K(1e-2), huge = COSZEN_TOLS * K(1e2);
+    R extkb = K(0.5), extkd = K(0.7);
+    if (veg_mask) {
+      R s = GW0 * exp(-xk[0] * vlaiw);
+      s = s + GW1 * exp(-xk[1] * vlaiw);
+      s = s + GW2 * exp(-xk[2] * vlaiw);
+      extkd = -log(s) / vlaiw;
+    }
+    if (veg_mask && coszen > tiny) extkb = xphi1 / coszen + xphi2;
+    if (coszen < tiny) extkb = K(1.0e5);
+    if (fabs(extkb - extkd) < K(0.001)) extkb = extkd + K(0.001);
+    t.rad_extkb[0] = extkb;
+    t.rad_extkd[0] = extkd;
+#pragma unroll
+    for (int b = 0; b < 3; b++) t.rad_extkbm[b] = K(0.0), t.rad_extkdm[b] = K(0.0);
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      if (veg_mask) t.rad_extkbm[b] = extkb * c1[b];
+      t.rad_extkdm[b] = extkd * c1[b];
+    }
+    R fb = spitter((int)t.met_doy[0], coszen, t.met_fsd[0] + t.met_fsd[1]);
+    t.rad_fbeam[0] = fb;
+    t.rad_fbeam[1] = fb;
+    if (coszen < huge) t.rad_fbeam[0] = K(0.0), t.rad_fbeam[1] = K(0.0);
+
+    // Albedo
+    R alb[3] = {K(0.0), K(0.0), K(0.0)};
+#pragma unroll
+    for (int b = 0; b < 3; b++) t.rad_rhocbm[b] = K(0.0), t.rad_rhocdf[b] = K(0.0);
+    {
+      const R alvo = K(0.95), aliro = K(0.70);
+      int iveg = t.veg_iveg[0];
+      R snowd = t.ssnow_snowd[0];
+      R sa = t.soil_albsoil[0];
+      if (iveg == LAKES)
+        sa = -K(0.022) * (MIN(K(275.0), MAX(K(260.0), t.ssnow_tgg[0])) - K(260.0)) + K(0.45);
+      if (snowd > K(1.0) && iveg == LAKES) sa = K(0.85);
+      R sfact = K(0.68);
+      if (sa <= K(0.14))
+        sfact = K(0.5);
+      else if (sa > K(0.14) && sa <= K(0.20))
+        sfact = K(0.62);
+      alb[1] = K(2.0) * sa / (K(1.0) + sfact);
+      alb[0] = sfact * alb[1];
+      R snrat = K(0.0), alir = K(0.0), alv = K(0.0);
+      if (snowd > K(1.0)) {
+        R tmp = snowd / MAX(t.ssnow_ssdnn[0], K(200.0));
+        snrat = MIN(K(1.0), tmp / (tmp + K(0.1)));
+        R fage = K(1.0) - K(1.0) / (K(1.0) + t.ssnow_snage[0]);
+        tmp = MAX(K(0.17365), coszen);
+        R fz = (tmp > K(0.5)) ? K(0.0) : K(1.5) / (K(1.0) + K(4.0) * tmp) - K(0.5);
+        R fzenm = MAX(K(0.0), fz);
+        tmp = alvo * (K(1.0) - K(0.2) * fage);
+        alv = K(0.4) * fzenm * (K(1.0) - tmp) + tmp;
+        tmp = aliro * (K(1.0) - K(0.5) * fage);
+        alir = K(0.4) * fzenm * (K(1.0) - tmp) + tmp;
+      }
+      alb[1] = MIN(aliro, (K(1.0) - snrat) * alb[1] + snrat * alir);
+      alb[0] = MIN(alvo, (K(1.0) - snrat) * alb[0] + snrat * alv);
+      if (t.soil_isoilm[0] == ICE_SOIL) {
+        alb[0] = alvo - K(0.05);
+        alb[1] = aliro - K(0.05);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      if (veg_mask) t.rad_rhocbm[b] = K(2.0) * extkb / (extkb + extkd) * rhoch[b];
+      t.rad_rhocdf[b] = rhoch[b] * K(2.0) *
+                        (GW0 * xk[0] / (xk[0] + extkd) + GW1 * xk[1] / (xk[1] + extkd) +
+                         GW2 * xk[2] / (xk[2] + extkd));
+    }
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      if (veg_mask) {
+        R d = MIN(t.rad_extkbm[b] * vlaiw, K(20.0));
+        t.rad_cexpkbm[b] = exp(-K(1.0) * d);
+      }
+      R d = t.rad_extkdm[b] * vlaiw;
+      t.rad_cexpkdm[b] = exp(-K(1.0) * d);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; b++) t.rad_reffbm[b] = alb[b], t.rad_reffdf[b] = alb[b];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      if (veg_mask) {
+        t.rad_reffdf[b] = t.rad_rhocdf[b] + (alb[b] - t.rad_rhocdf[b]) * p2(t.rad_cexpkdm[b]);
+        t.rad_reffbm[b] = t.rad_rhocbm[b] + (alb[b] - t.rad_rhocbm[b]) * p2(t.rad_cexpkbm[b]);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 3; b++) t.rad_albedo[b] = alb[b];
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+      if (veg_mask)
+        t.rad_albedo[b] = (K(1.0) - t.rad_fbeam[b]) * t.rad_reffdf[b] + t.rad_fbeam[b] * t.rad_reffbm[b];
+#pragma unroll
+    for (int b = 0; b < 3; b++) t.ssnow_albsoilsn[b] = alb[b];
+  }
+
+  // src/science/radiation/cbl_radiation.F90:30-218
+  CBM_HD void radiation(bool sunlit_veg) {
+    R vlaiw = t.canopy_vlaiw[0];
+    R extkb = t.rad_extkb[0], extkd = t.rad_extkd[0];
+    R extkn = PV(veg_extkn, 0);
+    R cf2n = exp(-extkn * vlaiw);
+    R transd = K(1.0);
+    if (vlaiw > LAI_THRESH) transd = exp(-extkd * vlaiw);
+    R dummy2 = MIN(extkb * vlaiw, K(30.));
+    R transb = exp(-dummy2);
+    t.rad_transd[0] = transd;
+    t.rad_transb[0] = transb;
+    R tvrad = t.met_tvrad[0];
+    R flpwb = SBOLTZ * p4(tvrad);
+    R flwv = EMLEAF * flpwb;
+    R flws = SBOLTZ * EMSOIL * p4(t.ssnow_tss[0]);
+    t.rad_flws[0] = flws;
+    R emair = t.met_fld[0] / flpwb;
+    R *gradis = t.rad_gradis, *qcan = t.rad_qcan;  // qcan[l + 2*b]
+    gradis[0] = gradis[1] = K(0.0);
+#pragma unroll
+    for (int k = 0; k < 6; k++) qcan[k] = K(0.0);
+    R rho = t.air_rho[0];
+    if (vlaiw > LAI_THRESH) {
+      gradis[0] = (K(4.0) * EMLEAF / (CAPP * rho)) * flpwb / (tvrad)*extkd *
+                  ((K(1.0) - transb * transd) / (extkb + extkd) + (transd - transb) / (extkb - extkd));
+      gradis[1] = (K(8.0) * EMLEAF / (CAPP * rho)) * flpwb / tvrad * extkd * (K(1.0) - transd) / extkd -
+                  gradis[0];
+      qcan[4] = (flws - flwv) * extkd * (transd - transb) / (extkb - extkd) +
+                (emair - EMLEAF) * extkd * flpwb * (K(1.0) - transd * transb) / (extkb + extkd);
+      qcan[5] = (K(1.0) - transd) * (flws + t.met_fld[0] - K(2.0) * flwv) - qcan[4];
+    }
+    gradis[0] = t.air_cmolar[0] * gradis[0];
+    gradis[1] = t.air_cmolar[0] * gradis[1];
+    gradis[0] = MAX(K(1.0e-3), gradis[0]);
+    gradis[1] = MAX(K(1.0e-3), gradis[1]);
+    R cf1 = K(0.0), cf3 = K(0.0);
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      if (sunlit_veg) {
+        R kdm = t.rad_extkdm[b], kbm = t.rad_extkbm[b], fb = t.rad_fbeam[b];
+        R rdf = t.rad_reffdf[b], rbm = t.rad_reffbm[b];
+        R ckd = t.rad_cexpkdm[b], ckb = t.rad_cexpkbm[b];
+        R tl = PV(veg_taul, b), rl = PV(veg_refl, b);
+        cf1 = (K(1.0) - transb * ckd) / (extkb + kdm);
+        cf3 = (K(1.0) - transb * ckb) / (extkb + kbm);
+        R fsd = t.met_fsd[b];
+        R tb = ((K(1) - transb) / extkb - (K(1) - p2(transb)) / (extkb + extkb));
+        qcan[0 + 2 * b] = fsd * ((K(1.0) - fb) * (K(1.0) - rdf) * kdm * cf1 + fb * (K(1.0) - rbm) * kbm * cf3 +
+                                 fb * (K(1.0) - tl - rl) * extkb * tb);
+        qcan[1 + 2 * b] = fsd * ((K(1.0) - fb) * (K(1.0) - rdf) * kdm * ((K(1.0) - ckd) / kdm - cf1) +
+                                 fb * (K(1.) - rbm) * kbm * ((K(1.0) - ckb) / kbm - cf3) -
+                                 fb * (K(1.0) - tl - rl) * extkb * tb);
+      }
+    }
+    R *fsd = t.met_fsd;
+    if (sunlit_veg) {
+      t.rad_qssabs[0] =
+          fsd[0] * (t.rad_fbeam[0] * (K(1.) - t.rad_reffbm[0]) * exp(-MIN(t.rad_extkbm[0] * vlaiw, K(20.))) +
+                    (K(1.) - t.rad_fbeam[0]) * (K(1.) - t.rad_reffdf[0]) *
+                        exp(-MIN(t.rad_extkdm[0] * vlaiw, K(20.)))) +
+          fsd[1] * (t.rad_fbeam[1] * (K(1.) - t.rad_reffbm[1]) * t.rad_cexpkbm[1] +
+                    (K(1.) - t.rad_fbeam[1]) * (K(1.) - t.rad_reffdf[1]) * t.rad_cexpkdm[1]);
+      t.rad_scalex[0] = (K(1.0) - transb * cf2n) / (extkb + extkn);
+      t.rad_fvlai[0] = (K(1.0) - transb) / extkb;
+      t.rad_fvlai[1] = vlaiw - t.rad_fvlai[0];
+    } else {
+      t.rad_qssabs[0] = (K(1.0) - t.ssnow_albsoilsn[0]) * fsd[0] + (K(1.0) - t.ssnow_albsoilsn[1]) * fsd[1];
+      t.rad_scalex[0] = K(0.0);
+      t.rad_fvlai[0] = K(0.0);
+      t.rad_fvlai[1] = vlaiw;
+    }
+    t.rad_scalex[1] = (K(1.0) - cf2n) / extkn - t.rad_scalex[0];
+#pragma unroll
+    for (int l = 0; l < 2; l++) t.rad_rniso[l] = qcan[l + 0] + qcan[l + 2] + qcan[l + 4];
+  }
+
+  // src/science/canopy/cbl_friction_vel.F90:112-166
+  CBM_HD static R psim(R zeta) {
+    const R gu = K(16.0), a_ = K(1.0), b = K(0.667), xc = K(5.0), d = K(0.35);
+    R z = K(0.5) + SIGN(K(0.5), zeta);
+    R stable = -a_ * zeta - b * (zeta - xc / d) * exp(-d * zeta) - b * xc / d;
+    R x = pow(K(1.0) + gu * fabs(zeta), K(0.25));
+    R unstable = log((K(1.0) + x * x) * p2(K(1.0) + x) / K(8)) - K(2.0) * atan(x) + PI_C * K(0.5);
+    return z * stable + (K(1.0) - z) * unstable;
+  }
+
+  // src/science/canopy/cbl_friction_vel.F90:168-221
+  CBM_HD static R psis(R zeta) {
+    const R gu = K(16.0), a_ = K(1.0), b = K(0.667), c = K(5.0), d = K(0.35);
+    R z = K(0.5) + SIGN(K(0.5), zeta);
+    R stzeta = MAX(K(0.), zeta);
+    R stable = -pow(K(1.) + K(2.) / K(3.) * a_ * stzeta, K(3.) / K(2.)) -
+               b * (stzeta - c / d) * exp(-d * stzeta) - b * c / d + K(1.);
+    R y = sqrt(K(1.0) + gu * fabs(zeta));
+    R unstable = K(2.0) * log((K(1) + y) * K(0.5));
+    return z * stable + (K(1.0) - z) * unstable;
+  }
+
+  // src/science/canopy/cbl_dryLeaf.F90:779-877
+  CBM_HD static R ej3x(R parx, R alpha, R convex, R x) {
+    return MAX(K(0.0), K(0.25) * ((alpha * parx + x - sqrt(p2(alpha * parx + x) - K(4.0) * convex * alpha * parx * x)) /
+                                  (K(2.0) * convex)));
+  }
+  CBM_HD static R ej4x(R parx, R alpha, R convex, R x) {
+    return MAX(K(0.0), (alpha * parx + x - sqrt(p2(alpha * parx + x) - K(4.0) * convex * alpha * parx * x)) /
+                           (K(2.0) * convex));
+  }
+  CBM_HD static R xvcmxt4(R x) {
+    return exp2(K(0.1) * x - K(2.5)) / ((K(1.0) + exp(K(0.3) * (K(13.0) - x))) * (K(1.0) + exp(K(0.3) * (x - K(36.0)))));
+  }
+  CBM_HD static R xvcmxt3(R x) {
+    const R EHaVc = K(73637.0), EHdVc = K(149252.0), EntropVc = K(486.0), xVccoef = K(1.17461);
+    R num = xVccoef * exp((EHaVc / (RGAS * TREFK)) * (K(1.) - TREFK / x));
+    R den = K(1.0) + exp((EntropVc * x - EHdVc) / (RGAS * x));
+    return MAX(K(0.0), num / den);
+  }
+  CBM_HD static R xejmxt3(R x) {
+    const R EHaJx = K(50300.0), EHdJx = K(152044.0), EntropJx = K(495.0), xjxcoef = K(1.16715);
+    R num = xjxcoef * exp((EHaJx / (RGAS * TREFK)) * (K(1.) - TREFK / x));
+    R den = K(1.0) + exp((EntropJx * x - EHdJx) / (RGAS * x));
+    return MAX(K(0.0), num / den);
+  }
+
+  // src/science/canopy/cbl_photosynthesis.F90:10-226, one leaf
+  CBM_HD static R photosynthesis_leaf(R csxz, R cx1z, R cx2z, R gswminz, R rdxz, R vcmxt3z,
+                                          R vcmxt4z, R vx3z, R vx4z, R gs_coeffz, R fwsoilz) {
+    const R effc4 = K(4000.0);
+    R coef0z, coef1z, coef2z, ciz, delcxz;
+    R anrubiscoz = K(0.0), anrubpz = K(0.0), ansinkz = K(0.0);
+    R gf = gswminz * fwsoilz / RGSWC;
+    coef2z = gf + gs_coeffz * (vcmxt3z - (rdxz - vcmxt4z));
+    coef1z = (K(1.0) - csxz * gs_coeffz) * (vcmxt3z + vcmxt4z - rdxz) + (gswminz * fwsoilz / RGSWC) * (cx1z - csxz) -
+             gs_coeffz * (vcmxt3z * cx2z / K(2.0) + cx1z * (rdxz - vcmxt4z));
+    coef0z = -(K(1.0) - csxz * gs_coeffz) * (vcmxt3z * cx2z / K(2.0) + cx1z * (rdxz - vcmxt4z)) -
+             (gswminz * fwsoilz / RGSWC) * cx1z * csxz;
+    if (fabs(coef2z) > K(1.0e-9) && fabs(coef1z) < K(1.0e-9)) {
+      ciz = K(99999.0);
+      anrubiscoz = K(99999.0);
+    }
+    if (fabs(coef2z) < K(1.e-9) && fabs(coef1z) >= K(1e-9)) {
+      ciz = -K(1.0) * coef0z / coef1z;
+      ciz = MAX(K(0.0), ciz);
+      anrubiscoz = vcmxt3z * (ciz - cx2z / K(2.0)) / (ciz + cx1z) + vcmxt4z - rdxz;
+    }
+    if (fabs(coef2z) >= K(1.e-9)) {
+      delcxz = p2(coef1z) - K(4.0) * coef0z * coef2z;
+      ciz = (-coef1z + sqrt(MAX(K(0.0), delcxz))) / (K(2.0) * coef2z);
+      ciz = MAX(K(0.0), ciz);
+      anrubiscoz = vcmxt3z * (ciz - cx2z / K(2.0)) / (ciz + cx1z) + vcmxt4z - rdxz;
+    }
+    coef2z = gf + gs_coeffz * (vx3z - (rdxz - vx4z));
+    coef1z = (K(1.0) - csxz * gs_coeffz) * (vx3z + vx4z - rdxz) + (gswminz * fwsoilz / RGSWC) * (cx2z - csxz) -
+             gs_coeffz * (vx3z * cx2z / K(2.0) + cx2z * (rdxz - vx4z));
+    coef0z = -(K(1.0) - csxz * gs_coeffz) * (vx3z * cx2z / K(2.0) + cx2z * (rdxz - vx4z)) -
+             (gswminz * fwsoilz / RGSWC) * cx2z * csxz;
+    ciz = K(99999.0);
+    anrubpz = K(99999.0);
+    if (fabs(coef2z) < K(1.e-9) && fabs(coef1z) >= K(1.e-9)) {
+      ciz = -K(1.0) * coef0z / coef1z;
+      ciz = MAX(K(0.0), ciz);
+      anrubpz = vx3z * (ciz - cx2z / K(2.0)) / (ciz + cx2z) + vx4z - rdxz;
+    }
+    if (fabs(coef2z) >= K(1.e-9)) {
+      delcxz = p2(coef1z) - K(4.0) * coef0z * coef2z;
+      ciz = (-coef1z + sqrt(MAX(K(0.0), delcxz))) / (K(2.0) * coef2z);
+      ciz = MAX(K(0.0), ciz);
+      anrubpz = vx3z * (ciz - cx2z / K(2.0)) / (ciz + cx2z) + vx4z - rdxz;
+    }
+    coef2z = gs_coeffz;
+    coef1z = gf + gs_coeffz * (rdxz - K(0.5) * vcmxt3z) + effc4 * vcmxt4z - gs_coeffz * csxz * effc4 * vcmxt4z;
+    coef0z = -(gswminz * fwsoilz / RGSWC) * csxz * effc4 * vcmxt4z + (rdxz - K(0.5) * vcmxt3z) * gswminz * fwsoilz / RGSWC;
+    if (fabs(coef2z) < K(1.0e-9) && fabs(coef1z) < K(1.0e-9)) {
+      ciz = K(99999.0);
+      ansinkz = K(99999.0);
+    }
+    if (fabs(coef2z) < K(1.e-9) && fabs(coef1z) >= K(1.e-9)) {
+      ciz = -K(1.0) * coef0z / coef1z;
+      ansinkz = ciz;
+    }
+    if (fabs(coef2z) >= K(1.e-9)) {
+      delcxz = p2(coef1z) - K(4.0) * coef0z * coef2z;
+      ciz = (-coef1z + sqrt(MAX(K(0.0), delcxz))) / (K(2.0) * coef2z);
+      ansinkz = ciz;
+    }
+    return MIN(MIN(anrubiscoz, anrubpz), ansinkz);
+  }
+
+  // src/science/soilsnow/cbl_remove_trans.F90:43-93
+  CBM_HD void transp_soil_water(R fevc, double *evapfbl) {
+    double dels = a.dels;
+    double diff_prev = 0.0;
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+      double xx = (double)fevc * dels / (double)HL * (double)PV(veg_froot, k) + diff_prev;
+      double dk = MAX(0.0, t.ssnow_wbliq[k] - 1.1 * PV(soil_swilt_vec, k)) * PV(soil_zse_vec, k) * 1000.0;
+      double xxd = xx - dk;
+      if (xxd > 0.0) {
+        evapfbl[k] = dk;
+        diff_prev = xxd;
+      } else {
+        evapfbl[k] = xx;
+        diff_prev = 0.0;
+      }
+    }
+  }
+
+  // working variables define_canopy shares with dryLeaf and wetLeaf
+  struct Can {
+    R cansat, dsx, fwsoil, tlfx, tlfy, ecy, hcy, rny;
+    R gbhu[2], gbhf[2], csx[2], ghwet;
+    R sum_rad_rniso, sum_rad_gradis;
+  };
+
+  // src/science/canopy/cbl_dryLeaf.F90:10-666
+  CBM_HD void dryLeaf(Can &c, int iter) {
+    const R jtomol = K(4.6e-6);
+    R dels = (R)a.dels;
+    R vlaiw = t.canopy_vlaiw[0];
+    R rlam = t.air_rlam[0];
+    R gs_coeff[2] = {K(0.0), K(0.0)};
+    if (iter == 1) {
+      // fwsoil_calc_std, src/science/canopy/cbl_fwsoil.F90:13-38
+      R s = K(0.0);
+#pragma unroll
+      for (int k = 0; k < MS; k++) {
+        double r = (t.ssnow_wbliq[k] - PV(soil_swilt_vec, k)) / (PV(soil_sfc_vec, k) - PV(soil_swilt_vec, k));
+        R term = PV(veg_froot, k) * MAX(K(1.0e-9), MIN(K(1.0), (R)r));
+        s = (k == 0) ? term : s + term;
+      }
+      R rwater = MAX(K(1.0e-9), s);
+      c.fwsoil = MAX(K(1.0e-9), MIN(K(1.0), PV(veg_vbeta, 0) * rwater));
+      t.canopy_fwsoil[0] = c.fwsoil;
+    }
+    R fwsoil = c.fwsoil;
+    R gswmin[2];
+    R vgswmin = PV(veg_gswmin, 0);
+    gswmin[0] = MAX(K(1.e-6), t.rad_scalex[0] * vgswmin);
+    gswmin[1] = MAX(K(1.e-6), t.rad_scalex[1] * vgswmin);
+    R gh[2] = {K(1.0e-3), K(1.0e-3)}, ghr[2] = {K(1.0e-3), K(1.0e-3)};
+    R rdx[2] = {K(0.0), K(0.0)}, anx[2] = {K(0.0), K(0.0)}, an_y[2] = {K(0.0), K(0.0)},
+      rdy[2] = {K(0.0), K(0.0)};
+    R rnx = c.sum_rad_rniso, abs_deltlf = K(999.0);
+    R hcx = K(0.0), ecx = c.sum_rad_rniso;
+    c.hcy = K(0.0);
+    R tlfxx = c.tlfx;
+    R psycst[2] = {t.air_psyc[0], t.air_psyc[0]};
+    R vcmxt3[2] = {K(0), K(0)}, vcmxt4[2] = {K(0), K(0)}, vx3[2] = {K(0), K(0)}, vx4[2] = {K(0), K(0)};
+    R cx1 = K(0.0), cx2 = K(0.0), deltlf = K(0.0);
+    double oldevapfbl[MS];
+    t.canopy_fevc[0] = 0.0;
+#pragma unroll
+    for (int k = 0; k < MS; k++) t.ssnow_evapfbl[k] = 0.0;
+    c.ghwet = K(1.0e-3);
+    t.canopy_fevw[0] = K(0.0);
+    t.canopy_fhvw[0] = K(0.0);
+    R sum_gbh = (c.gbhu[0] + c.gbhf[0]) + (c.gbhu[1] + c.gbhf[1]);
+    if (vlaiw <= LAI_THRESH) {
+      rnx = K(0.0);
+      ecx = K(0.0);
+      c.ecy = ecx;
+      abs_deltlf = K(0.0);
+      c.rny = rnx;
+    }
+    R deltlfy = abs_deltlf;
+    const R tvair = t.met_tvair[0], tk = t.met_tk[0];
+    const R fvl0 = t.rad_fvlai[0], fvl1 = t.rad_fvlai[1];
+    const bool leaves = fvl0 + fvl1 > LAI_THRESH;
+    for (int k = 1; k <= 20; k++) {
+      bool active = (vlaiw > LAI_THRESH && abs_deltlf > K(0.1));
+      // a tile that has converged keeps every value: the remaining passes of
+      // the reference's fixed-count loop change nothing for it
+      if (!active && k > 1) break;
+      if (active) {
+        npass++;
+        R tlfx = c.tlfx;
+        R dleaf = PV(veg_dleaf, 0);
+        c.ghwet = K(2.0) * sum_gbh;
+        R gras = MAX(K(1.0e-6), K(1.595E8) * fabs(tlfx - tvair) * p3(dleaf));
+        R g25 = pow(gras, K(0.25));
+        c.gbhf[0] = fvl0 * t.air_cmolar[0] * K(0.5) * DHEAT * (g25) / dleaf;
+        c.gbhf[1] = fvl1 * t.air_cmolar[0] * K(0.5) * DHEAT * (g25) / dleaf;
+        c.gbhf[0] = MAX(K(1.e-6), c.gbhf[0]);
+        c.gbhf[1] = MAX(K(1.e-6), c.gbhf[1]);
+#pragma unroll
+        for (int l = 0; l < 2; l++) {
+          gh[l] = K(2.0) * (c.gbhu[l] + c.gbhf[l]);
+          ghr[l] = t.rad_gradis[l] + gh[l];
+        }
+        R frac4 = PV(veg_frac4, 0), vcmax = PV(veg_vcmax, 0);
+        R temp = xvcmxt3(tlfx) * vcmax * (K(1.0) - frac4);
+        vcmxt3[0] = t.rad_scalex[0] * temp;
+        vcmxt3[1] = t.rad_scalex[1] * temp;
+        temp = xvcmxt4(tlfx - TFRZ) * vcmax * frac4;
+        vcmxt4[0] = t.rad_scalex[0] * temp;
+        vcmxt4[1] = t.rad_scalex[1] * temp;
+        temp = xejmxt3(tlfx) * PV(veg_ejmax, 0) * (K(1.0) - frac4);
+        R ejmxt3_0 = t.rad_scalex[0] * temp, ejmxt3_1 = t.rad_scalex[1] * temp;
+        R tdiff = tlfx - TREFK;
+        R conkct = PV(veg_conkc0, 0) * exp((PV(veg_ekc, 0) / (RGAS * TREFK)) * (K(1.0) - TREFK / tlfx));
+        R conkot = PV(veg_conko0, 0) * exp((PV(veg_eko, 0) / (RGAS * TREFK)) * (K(1.0) - TREFK / tlfx));
+        tlfxx = tlfx;
+        cx1 = conkct * (K(1.0) + K(0.21) / conkot);
+        cx2 = K(2.0) * GAM0 * (K(1.0) + GAM1 * tdiff + GAM2 * tdiff * tdiff);
+        R alpha = PV(veg_alpha, 0), convex = PV(veg_convex, 0);
+        R q0 = t.rad_qcan[0], q1 = t.rad_qcan[1];
+        vx3[0] = ej3x(q0 * jtomol * (K(1.0) - frac4), alpha, convex, ejmxt3_0);
+        vx3[1] = ej3x(q1 * jtomol * (K(1.0) - frac4), alpha, convex, ejmxt3_1);
+        vx4[0] = ej4x(q0 * jtomol * frac4, alpha, convex, vcmxt4[0]);
+        vx4[1] = ej4x(q1 * jtomol * frac4, alpha, convex, vcmxt4[1]);
+        R cfrd = PV(veg_cfrd, 0);
+        rdx[0] = (cfrd * vcmxt3[0] + cfrd * vcmxt4[0]);
+        rdx[1] = (cfrd * vcmxt3[1] + cfrd * vcmxt4[1]);
+        R a1 = PV(veg_a1gs, 0) / (K(1.0) + c.dsx / PV(veg_d0gs, 0));
+        gs_coeff[0] = (fwsoil / (c.csx[0] - K(0.0))) * a1;
+        gs_coeff[1] = (fwsoil / (c.csx[1] - K(0.0))) * a1;
+      }
+      anx[0] = anx[1] = K(0.0);
+      if (active && leaves) {
+        if (fvl0 > LAI_THRESH)
+          anx[0] = photosynthesis_leaf(c.csx[0], cx1, cx2, gswmin[0], rdx[0], vcmxt3[0], vcmxt4[0], vx3[0],
+                                       vx4[0], gs_coeff[0], fwsoil);
+        if (fvl1 > LAI_THRESH)
+          anx[1] = photosynthesis_leaf(c.csx[1], cx1, cx2, gswmin[1], rdx[1], vcmxt3[1], vcmxt4[1], vx3[1],
+                                       vx4[1], gs_coeff[1], fwsoil);
+      }
+      if (active) {
+#pragma unroll
+        for (int l = 0; l < 2; l++) {
+          if (t.rad_fvlai[l] > LAI_THRESH) {
+            c.csx[l] = t.met_ca[0] - RGBWC * anx[l] / (c.gbhu[l] + c.gbhf[l]);
+            c.csx[l] = MAX(K(1.0e-4), c.csx[l]);
+            t.canopy_gswx[l] = MAX(K(1.e-3), gswmin[l] * fwsoil + MAX(K(0.0), RGSWC * gs_coeff[l] * anx[l]));
+            R gw = K(1.0) / (K(1.0) / t.canopy_gswx[l] + K(1.0) / (K(1.075) * (c.gbhu[l] + c.gbhf[l])));
+            gw = MAX(gw, K(0.00001));
+            psycst[l] = t.air_psyc[0] * (ghr[l] / gw);
+          }
+        }
+        R dsatdk = t.air_dsatdk[0], dtv = tvair - tk;
+        R cr = CAPP * RMAIR;
+        ecx = (dsatdk * (t.rad_rniso[0] - cr * dtv * t.rad_gradis[0]) + cr * t.met_dva[0] * ghr[0]) /
+                  (dsatdk + psycst[0]) +
+              (dsatdk * (t.rad_rniso[1] - cr * dtv * t.rad_gradis[1]) + cr * t.met_dva[0] * ghr[1]) /
+                  (dsatdk + psycst[1]);
+        R fwet = t.canopy_fwet[0];
+        R local_fevc = (K(1.0) - fwet) * ecx;
+        if (local_fevc > K(0.0)) {
+          transp_soil_water(local_fevc, t.ssnow_evapfbl);
+          double s = t.ssnow_evapfbl[0];
+#pragma unroll
+          for (int q = 1; q < MS; q++) s = s + t.ssnow_evapfbl[q];
+          t.canopy_fevc[0] = s * (double)rlam / (double)a.dels;
+          ecx = (R)(t.canopy_fevc[0] / (double)(K(1.0) - fwet));
+        }
+        hcx = (c.sum_rad_rniso - ecx - cr * dtv * c.sum_rad_gradis) * (gh[0] + gh[1]) / (ghr[0] + ghr[1]);
+        c.tlfx = tvair + hcx / (cr * (gh[0] + gh[1]));
+        rnx = c.sum_rad_rniso - cr * (c.tlfx - tk) * c.sum_rad_gradis;
+        c.dsx = t.met_dva[0] + dsatdk * (c.tlfx - tvair);
+        c.dsx = MAX(c.dsx, K(0.0));
+        deltlf = tlfxx - c.tlfx;
+        abs_deltlf = fabs(deltlf);
+      }
+      if (abs_deltlf < fabs(deltlfy)) {
+        deltlfy = deltlf;
+        c.tlfy = c.tlfx;
+        c.rny = rnx;
+        c.hcy = hcx;
+        c.ecy = ecx;
+        rdy[0] = rdx[0];
+        rdy[1] = rdx[1];
+        an_y[0] = anx[0];
+        an_y[1] = anx[1];
+#pragma unroll
+        for (int q = 0; q < MS; q++) oldevapfbl[q] = t.ssnow_evapfbl[q];
+      }
+      if (abs_deltlf > K(0.1)) {
+        R w = K(0.5) * ((R)(k - 5 > 0 ? k - 5 : 0) / ((R)k - K(4.9999)));
+        c.tlfx = w * tlfxx + (K(1.0) - w) * c.tlfx;
+      }
+      if (k == 1) {
+        c.tlfy = c.tlfx;
+        c.rny = rnx;
+        c.hcy = hcx;
+        c.ecy = ecx;
+        rdy[0] = rdx[0];
+        rdy[1] = rdx[1];
+        an_y[0] = anx[0];
+        an_y[1] = anx[1];
+#pragma unroll
+        for (int q = 0; q < MS; q++) oldevapfbl[q] = t.ssnow_evapfbl[q];
+      }
+    }
+    R fwet = t.canopy_fwet[0];
+    t.canopy_fevc[0] = (K(1.0) - fwet) * c.ecy;
+    if (c.ecy > K(0.0) && fwet < K(1.0)) {
+      if (fabs(c.ecy - ecx) > K(1.0e-6)) {
+        double s = oldevapfbl[0];
+#pragma unroll
+        for (int q = 1; q < MS; q++) s = s + oldevapfbl[q];
+        if (!(fabs(t.canopy_fevc[0] - (s * (double)rlam / (double)a.dels)) > 1.0e-4)) {
+#pragma unroll
+          for (int q = 0; q < MS; q++) t.ssnow_evapfbl[q] = oldevapfbl[q];
+        }
+      }
+    }
+    t.canopy_frday[0] = K(12.0) * (rdy[0] + rdy[1]);
+    t.canopy_fpn[0] = MIN(-K(12.0) * (an_y[0] + an_y[1]), t.canopy_frday[0]);
+  }
+
+  // src/science/canopy/cbl_wetleaf.F90:9-111
+  CBM_HD void wetLeaf(Can &c) {
+    R dels = (R)a.dels;
+    c.ghwet = K(1.0e-3);
+    R gwwet, ghrwet;
+    t.canopy_fevw[0] = K(0.0);
+    t.canopy_fhvw[0] = K(0.0);
+    R sum_gbh = (c.gbhu[0] + c.gbhf[0]) + (c.gbhu[1] + c.gbhf[1]);
+    if (t.canopy_vlaiw[0] > LAI_THRESH) {
+      R rlam = t.air_rlam[0], dsatdk = t.air_dsatdk[0], psyc = t.air_psyc[0];
+      R srn = c.sum_rad_rniso, sg = c.sum_rad_gradis, fwet = t.canopy_fwet[0];
+      c.ghwet = K(2.0) * sum_gbh;
+      gwwet = K(1.075) * sum_gbh;
+      ghrwet = sg + c.ghwet;
+      R ccfevw = MIN(t.canopy_cansto[0] * rlam / dels, K(2.0) / (K(1440.0) / (dels / K(60.0))) * rlam);
+      R dtv = t.met_tvair[0] - t.met_tk[0];
+      R cr = CAPP * RMAIR;
+      R num = dsatdk * (srn - cr * dtv * sg) + cr * t.met_dva[0] * ghrwet;
+      R den = dsatdk + psyc * ghrwet / gwwet;
+      t.canopy_fevw[0] = MIN(fwet * num / den, ccfevw);
+      t.canopy_fevw_pot[0] = num / den;
+      t.canopy_fhvw[0] = fwet * (srn - cr * (c.tlfy - t.met_tk[0]) * sg) - t.canopy_fevw[0];
+    }
+  }
+
+  // src/science/canopy/cbl_pot_evap_snow.F90:79-167 (HDM)
+  CBM_HD R humidity_deficit_method(R dq, R dqu) {
+    if (t.ssnow_snowd[0] > K(1.0) || t.ssnow_tgg[0] == TFRZ) {
+      dq = MAX(-K(0.1e-3), dq);
+      dqu = MAX(-K(0.1e-3), dqu);
+    }
+    (void)dqu;
+    return t.air_rho[0] * t.air_rlam[0] * dq / t.ssnow_rtsoil[0];
+  }
+
+  // src/science/canopy/cbl_latent_heat.F90:15-285
+  CBM_HD void latent_heat_flux() {
+    R dels = (R)a.dels;
+    R potev = t.ssnow_potev[0], rlam = t.air_rlam[0], snowd = t.ssnow_snowd[0];
+    if (potev < K(0.)) t.ssnow_wetfac[0] = K(1.0);
+    R wetfac = t.ssnow_wetfac[0];
+    double fess = (double)(wetfac * potev);
+    R pudsto = t.ssnow_pudsto[0];
+    R pwet = MAX(K(0.), MIN(K(0.2), pudsto / MAX(K(1.), (R)PV(ssnow_pudsmx, 0))));
+    fess = fess * (double)(K(1.) - pwet);
+    R frescale = (R)a.zse[0] * DENSITY_LIQ * rlam / dels;
+    if (snowd < K(0.1) && fess > 0.) {
+      R flower = (R)t.ssnow_wb[0] - PV(soil_swilt, 0) / K(2.0);
+      R fupper = MAX(K(0.), flower * frescale - (R)t.ssnow_evapfbl[0] * rlam / dels);
+      fess = MIN(fess, (double)fupper);
+      fupper = (R)(t.ssnow_wb[0] - t.ssnow_wbice[0] / (double)FROZEN_LIMIT) * frescale;
+      fupper = MAX(fupper, K(0.));
+      fess = MIN(fess, (double)fupper);
+    }
+    R cls = K(1.);
+    if (snowd >= K(0.1)) {
+      cls = K(1.1335);
+      fess = (double)(cls * potev);
+    }
+    if (snowd < K(0.1) && potev < K(0.) && t.ssnow_tss[0] < TFRZ) {
+      cls = K(1.1335);
+      fess = (double)(cls * potev);
+    }
+    if (snowd >= K(0.1) && potev > K(0.)) {
+      cls = K(1.1335);
+      fess = (double)MIN((wetfac * potev) * cls, snowd / dels * rlam * cls);
+    }
+    t.ssnow_cls[0] = cls;
+    R fesp = MIN(pudsto / dels * rlam, MAX(pwet * potev, K(0.)));
+    t.canopy_fess[0] = fess;
+    t.canopy_fesp[0] = fesp;
+    t.canopy_fes[0] = fess + (double)fesp;
+  }
+
+  // src/science/canopy/cbl_within_canopy.F90:10-159
+  CBM_HD void within_canopy(Can &c, R rt0, R &qstvair) {
+    R cmolar = t.air_cmolar[0];
+    R rrbw = ((c.gbhu[0] + c.gbhf[0]) + (c.gbhu[1] + c.gbhf[1])) / cmolar;
+    R rrsw = (t.canopy_gswx[0] + t.canopy_gswx[1]) / cmolar;
+    R fix_eqn = t.ssnow_cls[0] * rt0 / (rt0 + K(0.0));
+    if (t.ssnow_potev[0] > K(0.)) fix_eqn = fix_eqn * t.ssnow_wetfac[0];
+    R fix_eqn2 = rt0 / (rt0 + K(0.0));
+    if (PV(veg_meth, 0) > K(0) && t.canopy_vlaiw[0] > LAI_THRESH && t.rough_hruff[0] > t.rough_z0soilsn[0]) {
+      R epsi = t.air_epsi[0], rt1 = t.rough_rt1[0], rlam = t.air_rlam[0];
+      R rho = t.air_rho[0];
+      R e1 = (K(1.) + epsi) * rrsw + rrbw;
+      R dmah = (rt0 + fix_eqn2 * rt1) * e1 + epsi * (rt0 * rt1) * (rrbw * rrsw);
+      R dmbh = (-rlam / CAPP) * (rt0 * rt1) * (rrbw * rrsw);
+      R dmch = e1 * rt0 * rt1 * (t.canopy_fhv[0] + t.canopy_fhs[0]) / (rho * CAPP);
+      R dmae = (-epsi * CAPP / rlam) * (rt0 * rt1) * (rrbw * rrsw);
+      R dmbe = (rt0 + fix_eqn * rt1) * e1 + (rt0 * rt1) * (rrbw * rrsw);
+      R dmce = e1 * rt0 * rt1 * ((R)t.canopy_fev[0] + (R)(t.canopy_fes[0] / (double)t.ssnow_cls[0])) / (rho * rlam);
+      R tk = t.met_tk[0], tss = t.ssnow_tss[0];
+      R det = dmah * dmbe - dmae * dmbh + K(1.0e-12);
+      R tvair = tk + (dmbe * dmch - dmbh * dmce) / det;
+      R lo = MIN(tss, tk) - K(5.0), hi = MAX(tss, tk) + K(5.0);
+      tvair = MAX(tvair, lo);
+      tvair = MIN(tvair, hi);
+      t.met_tvair[0] = tvair;
+      R qv = t.met_qv[0];
+      R qvair = qv + (dmah * dmce - dmae * dmch) / det;
+      qvair = MAX(K(0.0), qvair);
+      lo = MIN(t.ssnow_qstss[0], qv);
+      hi = MAX(t.ssnow_qstss[0], qv);
+      qvair = MAX(qvair, lo);
+      qvair = MIN(qvair, hi);
+      t.met_qvair[0] = qvair;
+      qstvair = qsatfjh(tvair - TFRZ, t.met_pmb[0]);
+      t.met_dva[0] = (qstvair - qvair) * RMAIR / RMH2O * t.met_pmb[0] * K(100.);
+    }
+  }
+
+  // src/science/canopy/cable_canopy.F90:10-1048
+  CBM_HD void define_canopy(bool sunlit_veg) {
+    Can c;
+    R dels = (R)a.dels;
+    R vlaiw = t.canopy_vlaiw[0];
+    bool veg = vlaiw > LAI_THRESH;
+    t.canopy_cansto[0] = t.canopy_oldcansto[0];
+    c.cansat = PV(veg_canst1, 0) * vlaiw;
+    {  // Surf_wetness_fact, src/science/canopy/cbl_SurfaceWetness.F90:10-79
+      R precip = t.met_precip[0], psn = t.met_precip_sn[0];
+      R upper = K(4.0) * MIN(dels, K(1800.0)) / (K(60.0) * K(1440.0));
+      R ftemp = MIN(precip - psn, upper);
+      R lower = c.cansat - t.canopy_cansto[0];
+      upper = MAX(lower, K(0.0));
+      R wcint = (ftemp > K(0.0) && t.met_tk[0] > TFRZ) ? MIN(upper, ftemp) : K(0.0);
+      t.canopy_wcint[0] = wcint;
+      t.canopy_through[0] = psn + MIN(precip - psn, MAX(K(0.0), precip - psn - wcint));
+      t.canopy_cansto[0] = t.canopy_cansto[0] + wcint;
+      t.canopy_fwet[0] = MAX(K(0.0), MIN(K(0.9), K(0.8) * t.canopy_cansto[0] / MAX(c.cansat, K(0.01))));
+      t.ssnow_satfrac[0] = 1.0e-8;
+      t.ssnow_rh_srf[0] = 1.0;
+      // initialize_wetfac, src/science/canopy/cbl_init_wetfac_mod.F90:9-116
+      R swilt = PV(soil_swilt, 0);
+      R wilting_pt = swilt / K(2.0);
+      R num = (R)t.ssnow_wb[0] - wilting_pt;
+      R den = PV(soil_sfc, 0) - wilting_pt;
+      den = MAX(K(0.0830), den);
+      R wf = num / den;
+      wf = MIN(K(1.0), wf);
+      wf = MAX(K(0.0), wf);
+      if (t.ssnow_wbice[0] > 0.0) {
+        double ice_ratio = p2(t.ssnow_wbice[0] / t.ssnow_wb[0]);
+        double ice_factor = 1.0 - MIN(0.2, ice_ratio);
+        ice_factor = MAX(0.5, ice_factor);
+        wf = wf * (R)ice_factor;
+      }
+      if (t.ssnow_snowd[0] > K(0.1)) wf = K(0.9);
+      if (t.veg_iveg[0] == LAKES) {
+        if (t.met_tk[0] >= TFRZ + K(5.)) wf = K(1.0);
+        if (t.met_tk[0] < TFRZ + K(5.)) wf = K(0.7);
+      }
+      t.ssnow_wetfac[0] = K(0.5) * (wf + t.ssnow_owetfac[0]);
+    }
+    t.canopy_fevw_pot[0] = K(0.0);
+    t.canopy_gswx[0] = t.canopy_gswx[1] = K(1e-3);
+    c.gbhf[0] = c.gbhf[1] = K(1e-3);
+    c.gbhu[0] = c.gbhu[1] = K(1e-3);
+#pragma unroll
+    for (int k = 0; k < MS; k++) t.ssnow_evapfbl[k] = 0.0, t.ssnow_rex[k] = 0.0;
+    c.csx[0] = c.csx[1] = t.met_ca[0];
+    t.met_tvair[0] = t.met_tk[0];
+    t.met_qvair[0] = t.met_qv[0];
+    t.canopy_tv[0] = t.met_tvair[0];
+    t.canopy_fwsoil[0] = 1.0;
+    t.canopy_kthlitt[0] = 0.3;
+    t.canopy_dvlitt[0] = 3.1415841138194147e-05;
+    define_air();
+    R qstvair = qsatfjh(t.met_tvair[0] - TFRZ, t.met_pmb[0]);
+    t.met_dva[0] = (qstvair - t.met_qvair[0]) * RMAIR / RMH2O * t.met_pmb[0] * K(100.0);
+    c.dsx = t.met_dva[0];
+    c.dsx = MAX(c.dsx, K(0.0));
+    c.tlfx = t.met_tk[0];
+    c.tlfy = t.met_tk[0];
+    c.fwsoil = K(1.0);
+    R ortsoil = t.ssnow_rtsoil[0];
+    int isflag = t.ssnow_isflag[0];
+    t.ssnow_tss[0] = (R)(1 - isflag) * t.ssnow_tgg[0] + (R)isflag * t.ssnow_tggsn[0];
+    R tss = t.ssnow_tss[0];
+    R tss4 = p4(tss);
+    t.canopy_fes[0] = 0.;
+    t.canopy_fess[0] = 0.;
+    t.canopy_fesp[0] = 0.;
+    t.ssnow_potev[0] = K(0.);
+    t.canopy_fevw_pot[0] = K(0.);
+    t.canopy_fhs_cor[0] = K(0.0);
+    t.canopy_fns_cor[0] = K(0.0);
+    t.canopy_ga_cor[0] = K(0.0);
+    radiation(sunlit_veg);
+    t.canopy_zetar[0] = ZETA0;
+    t.canopy_zetar[1] = ZETPOS + K(1);
+    t.canopy_zetash[0] = ZETA0;
+    t.canopy_zetash[1] = ZETPOS + K(1);
+    c.sum_rad_rniso = t.rad_rniso[0] + t.rad_rniso[1];
+    c.sum_rad_gradis = t.rad_gradis[0] + t.rad_gradis[1];
+    R rt0 = K(0.0), rt1usc = K(0.0);
+    int iterplus = 0;
+    const R zuv = t.rough_zref_uv[0], ztq = t.rough_zref_tq[0], z0m = t.rough_z0m[0];
+    const R disp = t.rough_disp[0], zruffs = t.rough_zruffs[0], z0soilsn = t.rough_z0soilsn[0];
+    const R tvrad = t.met_tvrad[0], transd = t.rad_transd[0];
+    for (int iter = 1; iter <= 4; iter++) {
+      // comp_friction_vel, src/science/canopy/cbl_friction_vel.F90:19-108
+      R zr = t.canopy_zetar[0];
+      if (iter == 2) zr = t.canopy_zetar[1];
+      if (iter == 3) zr = t.canopy_zetar[2];
+      if (iter == 4) zr = t.canopy_zetar[3];
+      R psim_1 = psim(zr * zuv / ztq);
+      R rescale = VONK * MAX(t.met_ua[0], UMIN);
+      R z_eff = zuv / z0m;
+      R psim_2 = psim(zr * z0m / ztq);
+      R ll = rescale / (log(z_eff) - psim_1 + psim_2);
+      t.canopy_us[0] = MIN(MAX(K(1.e-6), ll), K(10.0));
+      R us = t.canopy_us[0];
+      R xx = K(0.5) + SIGN(K(0.5), ztq + disp - zruffs);
+      R mz = MAX(zruffs - disp, z0soilsn);
+      rt1usc = xx * (log(ztq / mz) - psis(zr) + psis(zr * (mz) / ztq)) / VONK;
+      const R rt_min = K(5.);
+      rt0 = MAX(rt_min, t.rough_rt0us[0] / us);
+      t.rough_rt1[0] = MAX(K(5.), (t.rough_rt1usa[0] + t.rough_rt1usb[0] + rt1usc) / us);
+      R rtsoil = veg ? rt0 : rt0 + t.rough_rt1[0];
+      rtsoil = MAX(rt_min, rtsoil);
+      if (rtsoil > K(2.) * ortsoil || rtsoil < K(0.5) * ortsoil) rtsoil = MAX(rt_min, K(0.5) * (rtsoil + ortsoil));
+      t.ssnow_rtsoil[0] = rtsoil;
+      if (veg) {
+        R dleaf = PV(veg_dleaf, 0), visc = t.air_visc[0];
+        R gbvtop = t.air_cmolar[0] * APOL * visc / PRANDT / dleaf *
+                   sqrt(us / MAX(t.rough_usuh[0], K(1.e-6)) * dleaf / visc) * pow(PRANDT, K(1.0) / K(3.0)) /
+                   PV(veg_shelrb, 0);
+        gbvtop = MAX(K(0.05), gbvtop);
+        R coexp = t.rough_coexp[0], extkb = t.rad_extkb[0];
+        c.gbhu[0] = gbvtop * (K(1.0) - exp(-MIN(vlaiw * (K(0.5) * coexp + extkb), K(20.0)))) / (extkb + K(0.5) * coexp);
+        c.gbhu[1] = (K(2.0) / coexp) * gbvtop * (K(1.0) - exp(-MIN(K(0.5) * coexp * vlaiw, K(20.0)))) - c.gbhu[0];
+      }
+      c.rny = c.sum_rad_rniso;
+      c.hcy = K(0.0);
+      c.ecy = c.rny - c.hcy;
+      dryLeaf(c, iter);
+      wetLeaf(c);
+      t.canopy_fev[0] = (R)(t.canopy_fevc[0] + (double)t.canopy_fevw[0]);
+      R fwet = t.canopy_fwet[0];
+      t.canopy_fhv[0] = (K(1.0) - fwet) * c.hcy + t.canopy_fhvw[0];
+      t.canopy_fnv[0] = (K(1.0) - fwet) * c.rny + t.canopy_fevw[0] + t.canopy_fhvw[0];
+      if (veg && t.rough_hruff[0] > z0soilsn) {
+        t.rad_lwabv[0] = CAPP * RMAIR * (c.tlfy - t.met_tk[0]) * c.sum_rad_gradis;
+        R q = t.rad_lwabv[0] / (K(2.0) * (K(1.0) - transd) * SBOLTZ * EMLEAF) + p4(tvrad);
+        if (q > K(0.0))
+          t.canopy_tv[0] = pow(q, K(0.25));
+        else
+          t.canopy_tv[0] = tvrad;
+      } else {
+        t.canopy_tv[0] = tvrad;
+      }
+      t.canopy_fns[0] = t.rad_qssabs[0] + transd * t.met_fld[0] + (K(1.0) - transd) * EMLEAF * SBOLTZ * p4(t.canopy_tv[0]) -
+                        EMSOIL * SBOLTZ * tss4;
+      t.ssnow_qstss[0] = qsatfjh(tss - TFRZ, t.met_pmb[0]);
+      {
+        R dq = t.ssnow_qstss[0] - t.met_qv[0];
+        R dqu = (R)t.ssnow_rh_srf[0] * t.ssnow_qstss[0] - t.met_qv[0];
+        t.ssnow_potev[0] = humidity_deficit_method(dq, dqu);
+      }
+      latent_heat_flux();
+      t.canopy_fhs[0] = t.air_rho[0] * CAPP * (tss - t.met_tvair[0]) / t.ssnow_rtsoil[0];
+      within_canopy(c, rt0, qstvair);
+      t.ssnow_qstss[0] = qsatfjh(tss - TFRZ, t.met_pmb[0]);
+      {
+        R dq = t.ssnow_qstss[0] - t.met_qvair[0];
+        R dqu = (R)t.ssnow_rh_srf[0] * t.ssnow_qstss[0] - t.met_qvair[0];
+        t.ssnow_potev[0] = humidity_deficit_method(dq, dqu);
+      }
+      latent_heat_flux();
+      t.canopy_fhs[0] = t.air_rho[0] * CAPP * (tss - t.met_tvair[0]) / t.ssnow_rtsoil[0];
+      R fes = (R)t.canopy_fes[0];
+      t.canopy_ga[0] = t.canopy_fns[0] - t.canopy_fhs[0] - fes;
+      t.canopy_fe[0] = t.canopy_fev[0] + fes;
+      t.canopy_fh[0] = t.canopy_fhv[0] + t.canopy_fhs[0];
+      if (t.ssnow_potev[0] >= K(0.))
+        t.ssnow_potev[0] = MAX(K(0.00001), t.ssnow_potev[0]);
+      else
+        t.ssnow_potev[0] = MIN(-K(0.0002), t.ssnow_potev[0]);
+      if (t.canopy_fevw_pot[0] >= K(0.))
+        t.canopy_fevw_pot[0] = MAX(K(0.000001), t.canopy_fevw_pot[0]);
+      else
+        t.canopy_fevw_pot[0] = MIN(-K(0.002), t.canopy_fevw_pot[0]);
+      if constexpr (FULL) {
+        t.canopy_rnet[0] = t.canopy_fnv[0] + t.canopy_fns[0];
+        t.canopy_rniso[0] = c.sum_rad_rniso + t.rad_qssabs[0] + transd * t.met_fld[0] +
+                            (K(1.0) - transd) * EMLEAF * SBOLTZ * p4(tvrad) - EMSOIL * SBOLTZ * p4(tvrad);
+        t.canopy_epot[0] = (t.canopy_fevw_pot[0] + t.ssnow_potev[0] / t.ssnow_cls[0]) * dels / t.air_rlam[0];
+        R rll = t.canopy_epot[0] * t.air_rlam[0] / dels;
+        if (rll == K(0)) rll = K(1.e-7);
+        t.canopy_wetfac_cs[0] = MAX(K(0.), MIN(K(1.0), t.canopy_fe[0] / rll));
+        if (t.canopy_wetfac_cs[0] <= K(0.))
+          t.canopy_wetfac_cs[0] =
+              MAX(K(0.), MIN(K(1.), MAX(t.canopy_fev[0] / t.canopy_fevw_pot[0], fes / t.ssnow_potev[0])));
+      }
+      // update_zetar, src/science/canopy/cbl_zetar.F90:13-159
+      if (iter < 4) {
+        iterplus = (iter + 1 > 2) ? iter + 1 : 2;
+        R z = -(VONK * GRAV * ztq * (t.canopy_fh[0] + K(0.07) * t.canopy_fe[0])) /
+              (t.air_rho[0] * CAPP * t.met_tk[0] * p3(us));
+        z = MIN(ZETPOS, z);
+        z = MAX(ZETNEG, z);
+        if (iterplus == 2) t.canopy_zetar[1] = z;
+        if (iterplus == 3) t.canopy_zetar[2] = z;
+        if (iterplus == 4) t.canopy_zetar[3] = z;
+      }
+    }
+    R us = t.canopy_us[0], ua = t.met_ua[0];
+    R rho = t.air_rho[0];
+    R rlam = t.air_rlam[0];
+    if constexpr (FULL) {
+      t.canopy_cduv[0] = us * us / p2(MAX(ua, UMIN));
+      R lai_min = MAX(LAI_THRESH, vlaiw);
+      R rsun = t.rad_fvlai[0] / lai_min, rshd = t.rad_fvlai[1] / lai_min;
+      R cc = rsun * t.canopy_gswx[0] + rshd * t.canopy_gswx[1];
+      R mincc = MAX(K(1.e-06), cc);
+      cc = (K(1.) - transd) * mincc;
+      R relm = (R)t.ssnow_wb[0] / PV(soil_sfc, 0);
+      R sc = transd * p2(K(0.01) * relm);
+      R surf = cc + sc;
+      if (t.soil_isoilm[0] == ICE_SOIL) surf = K(1.e6);
+      t.canopy_gswx_t[0] = surf;
+      R z4 = t.canopy_zetar[3];
+      t.canopy_cdtq[0] = t.canopy_cduv[0] * (log(zuv / z0m) - psim(z4 * zuv / ztq) + psim(z4 * z0m / ztq)) /
+                         (log(ztq / (K(0.1) * z0m)) - psis(z4) + psis(z4 * K(0.1) * z0m / ztq));
+      R tstar = -t.canopy_fh[0] / (rho * CAPP * us);
+      R qstar = -t.canopy_fe[0] / (rho * rlam * us * t.ssnow_cls[0]);
+      R hruff = t.rough_hruff[0];
+      R zscrn = MAX(z0m, K(2.0) - disp);
+      R zp = t.canopy_zetar[3];
+      (void)iterplus;
+      R ftemp = (log(ztq / zscrn) - psis(zp) + psis(zp * zscrn / ztq)) / VONK;
+      t.canopy_tscrn[0] = t.met_tk[0] - TFRZ - tstar * ftemp;
+      R term1 = K(0.), term2 = K(0.), term5 = K(0.), term3 = K(0.), r_sc = K(0.);
+      R zscl = MAX(z0soilsn, K(2.0));
+      R rghlai = t.canopy_rghlai[0];
+      R rt0us = t.rough_rt0us[0], rt1usa = t.rough_rt1usa[0], rt1usb = t.rough_rt1usb[0];
+      bool scr = (vlaiw > LAI_THRESH && hruff > K(0.01));
+      if (scr) {
+        if (disp > K(0.0)) {
+          term1 = exp(K(2) * CSW * rghlai * (K(1) - zscl / hruff));
+          term2 = exp(K(2) * CSW * rghlai * (K(1) - disp / hruff));
+          term5 = MAX(K(2.) / K(3.) * hruff / disp, K(1.));
+        }
+        term3 = A33 * A33 * CTL * K(2) * CSW * rghlai;
+        if (zscl < disp) {
+          r_sc = term5 * log(zscl / z0soilsn) * (exp(K(2) * CSW * rghlai) - term2) / term3;
+          r_sc = r_sc + term5 * log(disp / zscl) * (exp(K(2) * CSW * rghlai) - term1) / term3;
+        } else if (disp <= zscl && zscl < hruff) {
+          r_sc = rt0us + term5 * (term2 - term1) / term3;
+        } else if (hruff <= zscl && zscl < zruffs) {
+          r_sc = rt0us + rt1usa + term5 * (zscl - hruff) / (A33 * A33 * CTL * hruff);
+        } else if (zscl >= zruffs) {
+          r_sc = rt0us + rt1usa + rt1usb +
+                 (log((zscl - disp) / MAX(zruffs - disp, z0soilsn)) - psis((zscl - disp) * zp / ztq) +
+                  psis((zruffs - disp) * zp / ztq)) /
+                     VONK;
+        }
+        t.canopy_tscrn[0] =
+            tss + (t.met_tk[0] - tss) * MIN(K(1.), (r_sc / MAX(K(1.), rt0us + rt1usa + rt1usb + rt1usc))) - TFRZ;
+      }
+      R rsts = qsatfjh(t.canopy_tscrn[0], t.met_pmb[0]);
+      R wetfac = t.ssnow_wetfac[0], qv = t.met_qv[0];
+      R qtgnet = rsts * wetfac - qv;
+      R qsurf = (qtgnet > K(0.)) ? rsts * wetfac : K(0.1) * rsts * wetfac + K(0.9) * qv;
+      t.canopy_qmom[0] = rho * (us * us);
+      t.canopy_qscrn[0] = qv - qstar * ftemp;
+      if (scr)
+        t.canopy_qscrn[0] = qsurf + (qv - qsurf) * MIN(K(1.), (r_sc / MAX(K(1.), rt0us + rt1usa + rt1usb + rt1usc)));
+    }
+    t.canopy_dewmm[0] = -(MIN(K(0.0), t.canopy_fevw[0]) + (R)MIN(0.0, t.canopy_fevc[0])) * dels / rlam;
+    t.canopy_cansto[0] = t.canopy_cansto[0] + t.canopy_dewmm[0];
+    t.canopy_cansto[0] = MAX(t.canopy_cansto[0] - MAX(K(0.0), t.canopy_fevw[0]) * dels / rlam, K(0.0));
+    t.canopy_spill[0] = MAX(K(0.0), t.canopy_cansto[0] - c.cansat);
+    t.canopy_through[0] = t.canopy_through[0] + t.canopy_spill[0];
+    t.canopy_precis[0] = MAX(K(0.), t.canopy_through[0]);
+    t.canopy_cansto[0] = t.canopy_cansto[0] - t.canopy_spill[0];
+    t.canopy_delwc[0] = t.canopy_cansto[0] - t.canopy_oldcansto[0];
+    R tssn = t.ssnow_tss[0];
+    t.ssnow_dfn_dtg[0] = (-K(1.)) * K(4.) * EMSOIL * SBOLTZ * tss4 / tssn;
+    R rttsoil = t.ssnow_rtsoil[0];
+    t.ssnow_dfh_dtg[0] = rho * CAPP / rttsoil;
+    t.ssnow_dfe_ddq[0] = t.ssnow_wetfac[0] * rho * rlam * t.ssnow_cls[0] / rttsoil;
+    t.ssnow_ddq_dtg[0] = (RMH2O / RMAIR) / t.met_pmb[0] * TETENA * TETENB * TETENC / (p2(TETENC + tssn - TFRZ)) *
+                         exp(TETENB * (tssn - TFRZ) / (TETENC + tssn - TFRZ));
+    t.ssnow_dfe_dtg[0] = t.ssnow_dfe_ddq[0] * t.ssnow_ddq_dtg[0];
+    t.canopy_dgdtg[0] = (double)t.ssnow_dfn_dtg[0] - (double)t.ssnow_dfh_dtg[0] - (double)t.ssnow_dfe_dtg[0];
+    if constexpr (FULL) {
+      R dtl = c.tlfy - t.met_tk[0];
+      t.bal_drybal[0] = (c.ecy + c.hcy) - c.sum_rad_rniso + CAPP * RMAIR * dtl * c.sum_rad_gradis;
+      t.bal_wetbal[0] = t.canopy_fevw[0] + t.canopy_fhvw[0] - c.sum_rad_rniso * t.canopy_fwet[0] +
+                        CAPP * RMAIR * dtl * c.sum_rad_gradis * t.canopy_fwet[0];
+      R *qc = t.rad_qcan;
+      t.rad_swnet[0] = (qc[0] + qc[1]) + (qc[2] + qc[3]) + t.rad_qssabs[0];
+      t.rad_lwnet[0] = t.met_fld[0] - SBOLTZ * EMLEAF * p4(t.canopy_tv[0]) * (K(1) - transd) - t.rad_flws[0] * transd;
+      t.rad_rnet[0] = t.rad_swnet[0] + t.rad_lwnet[0];
+    }
+  }
+
+  // ===================================================================
+  // soil and snow; REAL(r_2) quantities and both tridiagonal solves in FP64
+
+  // src/science/soilsnow/cbl_trimb.F90:17-53
+  template <int KMAX>
+  CBM_HD static void trimb(const double *a_, const double *b, const double *c, double *rhs) {
+    double e[KMAX], temp[KMAX], g[KMAX];
+    e[0] = c[0] / b[0];
+#pragma unroll
+    for (int k = 1; k < KMAX - 1; k++) {
+      temp[k] = 1. / (b[k] - a_[k] * e[k - 1]);
+      e[k] = c[k] * temp[k];
+    }
+    g[0] = rhs[0] / b[0];
+#pragma unroll
+    for (int k = 1; k < KMAX - 1; k++) g[k] = (rhs[k] - a_[k] * g[k - 1]) * temp[k];
+    rhs[KMAX - 1] = (rhs[KMAX - 1] - a_[KMAX - 1] * g[KMAX - 2]) / (b[KMAX - 1] - a_[KMAX - 1] * e[KMAX - 2]);
+#pragma unroll
+    for (int k = KMAX - 2; k >= 0; k--) rhs[k] = g[k] - e[k] * rhs[k + 1];
+  }
+
+  // src/science/soilsnow/cbl_snowCheck.F90:9-100
+  CBM_HD void snowcheck() {
+    R snowd = t.ssnow_snowd[0];
+    R *ssdn = t.ssnow_ssdn, *tggsn = t.ssnow_tggsn, *sdepth = t.ssnow_sdepth, *smass = t.ssnow_smass;
+    if (snowd <= K(0.0)) {
+      t.ssnow_isflag[0] = 0;
+      ssdn[0] = ssdn[1] = ssdn[2] = K(120.0);
+      t.ssnow_ssdnn[0] = K(120.0);
+      tggsn[0] = tggsn[1] = tggsn[2] = TFRZ;
+      sdepth[0] = snowd / ssdn[0];
+      sdepth[1] = K(0.);
+      sdepth[2] = K(0.);
+      smass[0] = snowd;
+      smass[1] = K(0.0);
+      smass[2] = K(0.0);
+    } else if (snowd < SNMIN * t.ssnow_ssdnn[0]) {
+      if (t.ssnow_isflag[0] == 1) {
+        ssdn[0] = t.ssnow_ssdnn[0];
+        t.ssnow_tgg[0] = tggsn[0];
+      }
+      t.ssnow_isflag[0] = 0;
+      t.ssnow_ssdnn[0] = MIN(K(400.0), MAX(K(120.0), ssdn[0]));
+      R tm = MIN(TFRZ, t.ssnow_tgg[0]);
+      tggsn[0] = tggsn[1] = tggsn[2] = tm;
+      sdepth[0] = snowd / ssdn[0];
+      sdepth[1] = K(0.0);
+      sdepth[2] = K(0.0);
+      smass[0] = snowd;
+      smass[1] = K(0.0);
+      smass[2] = K(0.0);
+      ssdn[0] = ssdn[1] = ssdn[2] = t.ssnow_ssdnn[0];
+    } else {
+      if (t.ssnow_isflag[0] == 0) {
+        R tm = MIN(TFRZ, t.ssnow_tgg[0]);
+        tggsn[0] = tggsn[1] = tggsn[2] = tm;
+        ssdn[1] = ssdn[0];
+        ssdn[2] = ssdn[0];
+        R tw = t.ssnow_t_snwlr[0];
+        sdepth[0] = tw;
+        smass[0] = tw * ssdn[0];
+        smass[1] = (snowd - smass[0]) * K(0.4);
+        smass[2] = (snowd - smass[0]) * K(0.6);
+        sdepth[1] = smass[1] / ssdn[1];
+        sdepth[2] = smass[2] / ssdn[2];
+        t.ssnow_ssdnn[0] = (ssdn[0] * smass[0] + ssdn[1] * smass[1] + ssdn[2] * smass[2]) / snowd;
+      }
+      t.ssnow_isflag[0] = 1;
+    }
+  }
+
+  // src/science/soilsnow/cbl_snowDensity.F90:9-102
+  CBM_HD void snowdensity() {
+    R dels = (R)a.dels;
+    R *ssdn = t.ssnow_ssdn, *tggsn = t.ssnow_tggsn, *smass = t.ssnow_smass;
+    R snowd = t.ssnow_snowd[0];
+    int isflag = t.ssnow_isflag[0];
+    R tmin1 = MIN(TFRZ, t.ssnow_tgg[0]);
+    if (snowd > K(0.1) && isflag == 0) {
+      ssdn[0] = MIN(MAX_SSDN, MAX(K(120.0), ssdn[0] + dels * ssdn[0] * K(3.1e-6) *
+                                                     exp(-K(0.03) * (K(273.15) - tmin1) -
+                                                         (ssdn[0] >= K(150.0) ? K(0.046) : K(0.0)) * (ssdn[0] - K(150.0)))));
+      ssdn[0] = MIN(MAX_SSDN, ssdn[0] + dels * K(9.806) * ssdn[0] * K(0.75) * snowd /
+                                            (K(3.0e7) * exp(K(0.021) * ssdn[0] + K(0.081) * (K(273.15) - MIN(TFRZ, t.ssnow_tgg[0])))));
+      if (t.soil_isoilm[0] != 9) ssdn[0] = MIN(K(450.0), ssdn[0]);
+      t.ssnow_sconds[0] = MAX(K(0.2), MIN(K(2.876e-6) * p2(ssdn[0]) + K(0.074), MAX_SCONDS));
+      t.ssnow_sconds[1] = t.ssnow_sconds[0];
+      t.ssnow_sconds[2] = t.ssnow_sconds[0];
+      t.ssnow_ssdnn[0] = ssdn[0];
+      ssdn[1] = ssdn[0];
+      ssdn[2] = ssdn[0];
+    }
+    if (isflag == 1) {
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        ssdn[k] = ssdn[k] + dels * ssdn[k] * K(3.1e-6) *
+                                exp(-K(0.03) * (K(273.15) - MIN(TFRZ, tggsn[k])) -
+                                    (ssdn[k] >= K(150.0) ? K(0.046) : K(0.0)) * (ssdn[k] - K(150.0)));
+      R tw = t.ssnow_t_snwlr[0];
+      ssdn[0] = ssdn[0] + dels * K(9.806) * ssdn[0] * tw * ssdn[0] /
+                              (K(3.0e7) * exp(K(.021) * ssdn[0] + K(0.081) * (K(273.15) - MIN(TFRZ, tggsn[0]))));
+      ssdn[1] = ssdn[1] + dels * K(9.806) * ssdn[1] * (tw * ssdn[0] + K(0.5) * smass[1]) /
+                              (K(3.0e7) * exp(K(.021) * ssdn[1] + K(0.081) * (K(273.15) - MIN(TFRZ, tggsn[1]))));
+      ssdn[2] = ssdn[2] + dels * K(9.806) * ssdn[2] * (tw * ssdn[0] + smass[1] + K(0.5) * smass[2]) /
+                              (K(3.0e7) * exp(K(.021) * ssdn[2] + K(0.081) * (K(273.15) - MIN(TFRZ, tggsn[2]))));
+#pragma unroll
+      for (int k = 0; k < 3; k++) t.ssnow_sdepth[k] = smass[k] / ssdn[k];
+      t.ssnow_ssdnn[0] = (ssdn[0] * smass[0] + ssdn[1] * smass[1] + ssdn[2] * smass[2]) / snowd;
+#pragma unroll
+      for (int k = 0; k < 3; k++) t.ssnow_sconds[k] = MAX(K(0.2), MIN(K(2.876e-6) * p2(ssdn[k]) + K(0.074), MAX_SCONDS));
+    }
+  }
+
+  // src/science/soilsnow/cbl_snowAccum.F90:10-188
+  CBM_HD void snow_accum() {
+    R dels = (R)a.dels;
+    R *ssdn = t.ssnow_ssdn, *smass = t.ssnow_smass, *sdepth = t.ssnow_sdepth, *tggsn = t.ssnow_tggsn;
+    R psn = t.met_precip_sn[0];
+    int isflag = t.ssnow_isflag[0];
+    bool notice = t.soil_isoilm[0] != 9;
+    R osnowd = t.ssnow_osnowd[0];
+    if (t.canopy_precis[0] > K(0.0) && isflag == 0) {
+      t.ssnow_snowd[0] = MAX(t.ssnow_snowd[0] + psn, K(0.0));
+      t.canopy_precis[0] = t.canopy_precis[0] - psn;
+      ssdn[0] = MAX(K(120.0), ssdn[0] * osnowd / MAX(K(0.01), t.ssnow_snowd[0]) + K(120.0) * psn / MAX(K(0.01), t.ssnow_snowd[0]));
+      t.ssnow_ssdnn[0] = ssdn[0];
+      if (t.canopy_precis[0] > K(0.0) && t.ssnow_tgg[0] < TFRZ) {
+        R pr = t.canopy_precis[0];
+        t.ssnow_snowd[0] = MAX(t.ssnow_snowd[0] + pr, K(0.0));
+        R gz = (R)t.ssnow_gammzz[0];
+        t.ssnow_tgg[0] = t.ssnow_tgg[0] + pr * HLF / (gz + CSWAT * pr);
+        t.ssnow_dtmlt[0] = t.ssnow_dtmlt[0] + pr * HLF / (gz + CSWAT * pr);
+        ssdn[0] = MIN(MAX_SSDN, MAX(K(120.0), ssdn[0] * osnowd / MAX(K(0.01), t.ssnow_snowd[0]) +
+                                                  DENSITY_LIQ * pr / MAX(K(0.01), t.ssnow_snowd[0])));
+        if (notice) ssdn[0] = MIN(K(450.0), ssdn[0]);
+        t.canopy_precis[0] = K(0.0);
+        t.ssnow_ssdnn[0] = ssdn[0];
+      }
+    }
+    if (t.canopy_precis[0] > K(0.0) && isflag > 0) {
+      t.ssnow_snowd[0] = MAX(t.ssnow_snowd[0] + psn, K(0.0));
+      t.canopy_precis[0] = t.canopy_precis[0] - psn;
+      R osm = smass[0];
+      smass[0] = smass[0] + psn;
+      ssdn[0] = MAX(K(120.0), ssdn[0] * osm / smass[0] + K(120.0) * psn / smass[0]);
+      sdepth[0] = MAX(K(0.02), smass[0] / ssdn[0]);
+      if (t.canopy_precis[0] > K(0.0)) {
+        R pr = t.canopy_precis[0];
+        t.ssnow_snowd[0] = MAX(t.ssnow_snowd[0] + pr, K(0.0));
+        R sgamm = ssdn[0] * CGSNOW * sdepth[0];
+        osm = smass[0];
+        tggsn[0] = tggsn[0] + pr * HLF * osm / (sgamm * osnowd);
+        t.ssnow_dtmlt[0] = t.ssnow_dtmlt[0] + pr * HLF * osm / (sgamm * osnowd);
+        smass[0] = smass[0] + pr * osm / osnowd;
+        ssdn[0] = MAX(K(120.0), MIN(ssdn[0] * osm / smass[0] + DENSITY_LIQ * (K(1.0) - osm / smass[0]), MAX_SSDN));
+        if (notice) ssdn[0] = MIN(K(450.0), ssdn[0]);
+        sdepth[0] = smass[0] / ssdn[0];
+#pragma unroll
+        for (int k = 1; k < 3; k++) {
+          sgamm = ssdn[k] * CGSNOW * sdepth[k];
+          osm = smass[k];
+          tggsn[k] = tggsn[k] + pr * HLF * osm / (sgamm * osnowd);
+          smass[k] = smass[k] + pr * osm / osnowd;
+          ssdn[k] = MAX(K(120.0), MIN(ssdn[k] * osm / smass[k] + DENSITY_LIQ * (K(1.0) - osm / smass[k]), MAX_SSDN));
+          if (notice) ssdn[k] = MIN(K(450.0), ssdn[k]);
+          sdepth[k] = smass[k] / ssdn[k];
+        }
+        t.canopy_precis[0] = K(0.0);
+      }
+    }
+    double fsc = t.canopy_fess[0] + PV(canopy_fes_cor, 0);
+    t.canopy_segg[0] = (R)(fsc / (double)HL);
+    t.ssnow_evapsn[0] = K(0);
+    if (t.ssnow_cls[0] == K(1.1335)) {
+      t.ssnow_evapsn[0] = (R)((double)dels * fsc / (double)(HL + HLF));
+      R xxx = t.ssnow_evapsn[0];
+      isflag = t.ssnow_isflag[0];
+      if (isflag == 0 && fsc > 0.0) t.ssnow_evapsn[0] = MIN(t.ssnow_snowd[0], xxx);
+      if (isflag > 0 && fsc > 0.0) t.ssnow_evapsn[0] = MIN(K(0.9) * smass[0], xxx);
+      t.ssnow_snowd[0] = t.ssnow_snowd[0] - t.ssnow_evapsn[0];
+      if (isflag > 0) {
+        smass[0] = smass[0] - t.ssnow_evapsn[0];
+        sdepth[0] = MAX(K(0.02), smass[0] / ssdn[0]);
+      }
+      t.canopy_segg[0] = (HL + HLF) * (xxx - t.ssnow_evapsn[0]) / HL / dels;
+    }
+  }
+
+  // src/science/soilsnow/cbl_snowMelt.F90:9-120
+  CBM_HD R snow_melting() {
+    R dels = (R)a.dels;
+    R snowmlt = K(0.0);
+    int isflag = t.ssnow_isflag[0];
+    if (t.ssnow_snowd[0] > K(0.0) && isflag == 0 && t.ssnow_tgg[0] >= TFRZ) {
+      R gz = (R)t.ssnow_gammzz[0];
+      R snowflx = (t.ssnow_tgg[0] - TFRZ) * gz;
+      snowmlt = MIN(snowflx / HLF, t.ssnow_snowd[0]);
+      t.ssnow_dtmlt[0] = t.ssnow_dtmlt[0] + snowmlt * HLF / gz;
+      t.ssnow_snowd[0] = t.ssnow_snowd[0] - snowmlt;
+      t.ssnow_tgg[0] = t.ssnow_tgg[0] - snowmlt * HLF / gz;
+    }
+    if (t.ssnow_snowd[0] > K(0.0) && isflag > 0) {
+      R *ssdn = t.ssnow_ssdn, *smass = t.ssnow_smass, *sdepth = t.ssnow_sdepth, *tggsn = t.ssnow_tggsn;
+      bool notice = t.soil_isoilm[0] != 9;
+      R prev = K(0.0);   // smelt1(k-1) of the reference
+      R last = K(0.0);
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        R sgamm = ssdn[q] * CGSNOW * sdepth[q];
+        R snowflx = prev * HLF / dels;
+        tggsn[q] = tggsn[q] + (snowflx * dels + prev * CSWAT * (TFRZ - tggsn[q])) / (sgamm + CSWAT * prev);
+        R osm = smass[q];
+        smass[q] = smass[q] + prev;
+        ssdn[q] = MAX(K(120.0), MIN(ssdn[q] * osm / smass[q] + DENSITY_LIQ * (K(1.0) - osm / smass[q]), MAX_SSDN));
+        if (notice) ssdn[q] = MIN(K(450.0), ssdn[q]);
+        sdepth[q] = smass[q] / ssdn[q];
+        sgamm = smass[q] * CGSNOW;
+        R cur = K(0.0);
+        if (tggsn[q] > TFRZ) {
+          snowflx = (tggsn[q] - TFRZ) * sgamm;
+          cur = MIN(snowflx / HLF, K(0.6) * smass[q]);
+          t.ssnow_dtmlt[q] = t.ssnow_dtmlt[q] + cur * HLF / sgamm;
+          smass[q] = smass[q] - cur;
+          tggsn[q] = tggsn[q] - cur * HLF / sgamm;
+          sdepth[q] = smass[q] / ssdn[q];
+        }
+        prev = cur;
+        last = cur;
+      }
+      // smelt1(:,1) and smelt1(:,2) were zeroed by the next layer's pass
+      snowmlt = K(0.0) + K(0.0) + last;
+      t.ssnow_snowd[0] = t.ssnow_snowd[0] - snowmlt;
+    }
+    return snowmlt;
+  }
+
+  // src/science/soilsnow/cbl_snowl_adjust.F90:9-156
+  CBM_HD void snowl_adjust() {
+    if (t.ssnow_isflag[0] <= 0) return;
+    R *ssdn = t.ssnow_ssdn, *smass = t.ssnow_smass, *sdepth = t.ssnow_sdepth, *tggsn = t.ssnow_tggsn;
+    R tw = t.ssnow_t_snwlr[0], snowd = t.ssnow_snowd[0];
+    double excd, excm;
+    R osm;
+    if (sdepth[0] > tw) {
+      excd = (double)sdepth[0] - (double)tw;
+      excm = excd * (double)ssdn[0];
+      sdepth[0] = sdepth[0] - (R)excd;
+      smass[0] = smass[0] - (R)excm;
+      osm = smass[1];
+      smass[1] = MAX(K(0.01), smass[1] + (R)excm);
+      ssdn[1] = (R)MAX(120.0, MIN((double)MAX_SSDN, (double)ssdn[1] * osm / smass[1] + (double)ssdn[0] * excm / smass[1]));
+      sdepth[1] = smass[1] / ssdn[1];
+      tggsn[1] = (R)((double)tggsn[1] * osm / smass[1] + (double)tggsn[0] * excm / smass[1]);
+      smass[2] = MAX(K(0.01), snowd - smass[0] - smass[1]);
+    } else {
+      excd = (double)tw - (double)sdepth[0];
+      excm = excd * (double)ssdn[1];
+      osm = smass[0];
+      smass[0] = smass[0] + (R)excm;
+      sdepth[0] = tw;
+      ssdn[0] = (R)MAX(120.0, MIN((double)MAX_SSDN, (double)ssdn[0] * osm / smass[0] + (double)ssdn[1] * excm / smass[0]));
+      tggsn[0] = (R)((double)tggsn[0] * osm / smass[0] + (double)tggsn[1] * excm / smass[0]);
+      smass[1] = MAX(K(0.01), smass[1] - (R)excm);
+      sdepth[1] = smass[1] / ssdn[1];
+      smass[2] = MAX(K(0.01), snowd - smass[0] - smass[1]);
+    }
+    double frac = (double)smass[1] / MAX(0.02, (double)smass[2]);
+    double xfrac = 2.0 / 3.0 / frac;
+    if (xfrac > 1.0) {
+      excm = (xfrac - 1.0) * smass[1];
+      osm = smass[1];
+      smass[1] = MAX(K(0.01), smass[1] + (R)excm);
+      tggsn[1] = tggsn[1] * osm / smass[1] + tggsn[2] * (R)excm / smass[1];
+      ssdn[1] = MAX(K(120.0), MIN(MAX_SSDN, ssdn[1] * osm / smass[1] + ssdn[2] * (R)excm / smass[1]));
+      smass[2] = MAX(K(0.01), snowd - smass[0] - smass[1]);
+      sdepth[2] = MAX(K(0.02), smass[2] / ssdn[2]);
+    } else {
+      excm = (1 - xfrac) * smass[1];
+      smass[1] = MAX(K(0.01), smass[1] - (R)excm);
+      sdepth[1] = MAX(K(0.02), smass[1] / ssdn[1]);
+      osm = smass[2];
+      smass[2] = MAX(K(0.01), snowd - smass[0] - smass[1]);
+      tggsn[2] = tggsn[2] * osm / smass[2] + tggsn[1] * (R)excm / smass[2];
+      ssdn[2] = MAX(K(120.0), MIN(MAX_SSDN, ssdn[2] * osm / smass[2] + ssdn[1] * (R)excm / smass[2]));
+      sdepth[2] = smass[2] / ssdn[2];
+    }
+    t.ssnow_isflag[0] = 1;
+    t.ssnow_ssdnn[0] = (ssdn[0] * sdepth[0] + ssdn[1] * sdepth[1] + ssdn[2] * sdepth[2]) / (sdepth[0] + sdepth[1] + sdepth[2]);
+  }
+
+  // src/science/soilsnow/cbl_stempv.F90:13-221 with old_soil_conductivity,
+  // src/science/soilsnow/cbl_Oldconductivity.F90:7-59
+  CBM_HD void stempv() {
+    const double *zse = a.zse;
+    double dels = a.dels;
+    double at[9], bt[9], ct[9], coeff[10], ccnsw[MS];
+#pragma unroll
+    for (int r = 0; r < 9; r++) at[r] = 0.0, bt[r] = 1.0, ct[r] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 10; r++) coeff[r] = 0.0;
+    double ssat = PV(soil_ssat, 0);
+    bool ice = t.soil_isoilm[0] == 9;
+    double cnsd = PV(soil_cnsd, 0);
+    const double l60 = log(60.0), l250 = log(250.0);
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+      if (ice) {
+        ccnsw[k] = (double)SNOW_CCNSW;
+      } else {
+        double ew = (double)(R)(t.ssnow_wblf[k] * ssat);
+        double exp_arg = (ew * l60) + (t.ssnow_wbfice[k] * ssat * l250);
+        double f = MAX(1.0, sqrt(MIN(2.0, 0.5 * ssat / MIN(ew, 0.5 * ssat))));
+        if (exp_arg > 30)
+          ccnsw[k] = 1.5 * f;
+        else
+          ccnsw[k] = MIN(cnsd * exp(exp_arg), 1.5) * f;
+      }
+    }
+    int isflag = t.ssnow_isflag[0];
+    double coefa = 0.0, coefb = 0.0;
+    double *gammzz = t.ssnow_gammzz;
+    double css = PV(soil_css, 0), rhosoil = PV(soil_rhosoil, 0);
+    const double dgdtg = t.canopy_dgdtg[0], ga = (double)t.canopy_ga[0];
+    if (isflag == 0) {
+      double xx = MAX(0., (double)(t.ssnow_snowd[0] / t.ssnow_ssdnn[0]));
+      ccnsw[0] = (ccnsw[0] - 0.2) * (zse[0] / (zse[0] + xx)) + 0.2;
+#pragma unroll
+      for (int k = 3; k <= MS; k++) coeff[k + 2] = 2.0 / (zse[k - 2] / ccnsw[k - 2] + zse[k - 1] / ccnsw[k - 1]);
+      coeff[4] = 2.0 / ((zse[0] + xx) / ccnsw[0] + zse[1] / ccnsw[1]);
+      coefa = 0.0;
+      coefb = coeff[4];
+#pragma unroll
+      for (int k = 1; k <= MS; k++) {
+        double wblfsp = t.ssnow_wblf[k - 1];
+        double hcll = (double)t.soil_heat_cap_lower_limit[k - 1];
+        gammzz[k - 1] = MAX(hcll, (1.0 - ssat) * css * rhosoil +
+                                      ssat * (wblfsp * (double)CSWAT * (double)DENSITY_LIQ +
+                                              t.ssnow_wbfice[k - 1] * (double)CSICE * (double)DENSITY_ICE)) *
+                        zse[k - 1];
+        if (k == 1) gammzz[0] = gammzz[0] + (double)CGSNOW * (double)t.ssnow_snowd[0];
+        double dtg = dels / gammzz[k - 1];
+        at[k + 2] = -dtg * coeff[k + 2];
+        ct[k + 2] = -dtg * coeff[k + 3];
+        bt[k + 2] = 1.0 - at[k + 2] - ct[k + 2];
+      }
+      bt[3] = bt[3] - dgdtg * dels / gammzz[0];
+      t.ssnow_tgg[0] = (R)((double)t.ssnow_tgg[0] + (ga - (double)t.ssnow_tgg[0] * dgdtg) * dels / gammzz[0]);
+    } else {
+      R *ssdn = t.ssnow_ssdn, *sd = t.ssnow_sdepth, *sc = t.ssnow_sconds;
+#pragma unroll
+      for (int k = 0; k < 3; k++) sc[k] = MAX(K(0.2), MIN(K(2.876e-6) * p2(ssdn[k]) + K(0.074), MAX_SCONDS));
+      coeff[1] = 2.0 / ((double)(sd[0] / sc[0]) + (double)(sd[1] / sc[1]));
+      coeff[2] = 2.0 / ((double)(sd[1] / sc[1]) + (double)(sd[2] / sc[2]));
+      coeff[3] = 2.0 / ((double)(sd[2] / sc[2]) + zse[0] / ccnsw[0]);
+#pragma unroll
+      for (int k = 2; k <= MS; k++) coeff[k + 2] = 2.0 / (zse[k - 2] / ccnsw[k - 2] + zse[k - 1] / ccnsw[k - 1]);
+      coefa = coeff[1];
+      coefb = coeff[3];
+#pragma unroll
+      for (int k = 1; k <= 3; k++) {
+        double sgamm = (double)(ssdn[k - 1] * CGSNOW * sd[k - 1]);
+        double dtg = dels / sgamm;
+        at[k - 1] = -dtg * coeff[k - 1];
+        ct[k - 1] = -dtg * coeff[k];
+        bt[k - 1] = 1.0 - at[k - 1] - ct[k - 1];
+      }
+#pragma unroll
+      for (int k = 1; k <= MS; k++) {
+        double wblfsp = t.ssnow_wblf[k - 1];
+        double hcll = (double)t.soil_heat_cap_lower_limit[k - 1];
+        gammzz[k - 1] = MAX((1.0 - ssat) * css * rhosoil +
+                                ssat * (wblfsp * (double)CSWAT * (double)DENSITY_LIQ +
+                                        t.ssnow_wbfice[k - 1] * (double)CSICE * (double)DENSITY_ICE),
+                            hcll) *
+                        zse[k - 1];
+        double dtg = dels / gammzz[k - 1];
+        at[k + 2] = -dtg * coeff[k + 2];
+        ct[k + 2] = -dtg * coeff[k + 3];
+        bt[k + 2] = 1.0 - at[k + 2] - ct[k + 2];
+      }
+      double sgamm = (double)(ssdn[0] * CGSNOW * sd[0]);
+      bt[0] = bt[0] - dgdtg * dels / sgamm;
+      t.ssnow_tggsn[0] = (R)((double)t.ssnow_tggsn[0] + (ga - (double)t.ssnow_tggsn[0] * dgdtg) * dels / sgamm);
+    }
+    double rhs[9];
+#pragma unroll
+    for (int k = 0; k < 3; k++) rhs[k] = t.ssnow_tggsn[k];
+#pragma unroll
+    for (int k = 0; k < MS; k++) rhs[3 + k] = t.ssnow_tgg[k];
+    trimb<9>(at, bt, ct, rhs);
+#pragma unroll
+    for (int k = 0; k < 3; k++) t.ssnow_tggsn[k] = (R)rhs[k];
+#pragma unroll
+    for (int k = 0; k < MS; k++) t.ssnow_tgg[k] = (R)rhs[3 + k];
+    if constexpr (FULL) {
+      t.canopy_sghflux[0] = (R)(coefa * (double)(t.ssnow_tggsn[0] - t.ssnow_tggsn[1]));
+      t.canopy_ghflux[0] = (R)(coefb * (double)(t.ssnow_tgg[0] - t.ssnow_tgg[1]));
+    }
+  }
+
+  // src/science/soilsnow/cbl_remove_trans.F90:9-40
+  CBM_HD void remove_trans() {
+    if (t.canopy_fevc[0] < 0.0) {
+      t.canopy_fevw[0] = (R)((double)t.canopy_fevw[0] + t.canopy_fevc[0]);
+      t.canopy_fevc[0] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+      t.ssnow_wbliq[k] = t.ssnow_wbliq[k] - t.ssnow_evapfbl[k] / (PV(soil_zse_vec, k) * 1000.0);
+      t.ssnow_wb[k] = t.ssnow_wbliq[k] + t.ssnow_wbice[k];
+    }
+  }
+
+  // src/science/soilsnow/cbl_soilfreeze.F90:9-81
+  CBM_HD void soilfreeze() {
+    const double *zse = a.zse;
+    const double FL = (double)FROZEN_LIMIT, TF = (double)TFRZ, HF = (double)HLF;
+    double ssat = PV(soil_ssat, 0), css = PV(soil_css, 0), rhosoil = PV(soil_rhosoil, 0);
+    double base = (double)((R)(K(1.0) - (R)ssat) * (R)css * (R)rhosoil);
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+      double wb = t.ssnow_wb[k], wbice = t.ssnow_wbice[k], gz = t.ssnow_gammzz[k];
+      double tgg = t.ssnow_tgg[k];
+      if (tgg < TF && FL * wb - wbice > .001) {
+        double sf = MIN(FL * wb - wbice, (ssat - wb) / MAX((1.0 - 921.0 / 1000.0), 1.0E-3));
+        sf = MIN(MAX(0.0, sf) * zse[k] * 921.0, (TF - tgg) * gz / HF);
+        wbice = MIN(wbice + sf / (zse[k] * 921.0), FL * wb);
+        wb = wb + sf / (zse[k] * 921.0) - sf / (zse[k] * 1000.0);
+        double a1 = (double)t.soil_heat_cap_lower_limit[k];
+        double a2 = base + (wb - wbice) * (4.218e3 * 1000.0) + wbice * (2.100e3 * 921.0);
+        gz = MAX(a1, a2) * zse[k];
+        if (k == 0 && t.ssnow_isflag[0] == 0) gz = gz + (double)CGSNOW * (double)t.ssnow_snowd[0];
+        t.ssnow_tgg[k] = (R)(tgg + sf * HF / gz);
+      } else if (tgg > TF && wbice > 0.) {
+        double sm = MIN(wbice * zse[k] * 921.0, (tgg - TF) * gz / HF);
+        wbice = MAX(0.0, wbice - sm / (zse[k] * 921.0));
+        wb = wb - sm / (zse[k] * 921.0) + sm / (zse[k] * 1000.0);
+        double a1 = (double)t.soil_heat_cap_lower_limit[k];
+        double a2 = base + (wb - wbice) * (4.218e3 * 1000.0) + wbice * (2.100e3 * 921.0);
+        gz = MAX(a1, a2) * zse[k];
+        if (k == 0 && t.ssnow_isflag[0] == 0) gz = gz + (double)CGSNOW * (double)t.ssnow_snowd[0];
+        t.ssnow_tgg[k] = (R)(tgg - sm * HF / gz);
+      }
+      t.ssnow_wb[k] = wb;
+      t.ssnow_wbice[k] = wbice;
+      t.ssnow_gammzz[k] = gz;
+    }
+  }
+
+  // src/science/soilsnow/cbl_smoisturev.F90:11-444 (nmeth = -1)
+  CBM_HD void smoisturev() {
+    const double *zse = a.zse;
+    double dels = a.dels;
+    double at[MS], bt[MS], ct[MS], fluxh[MS + 1], delt[MS + 1], dtt[MS + 1];
+    double *wb = t.ssnow_wb, *wbice = t.ssnow_wbice, *wblf = t.ssnow_wblf;
+    double ssat = PV(soil_ssat, 0), hyds = PV(soil_hyds, 0);
+    double e1 = (double)PV(soil_i2bp3, 0) - 1;
+#pragma unroll
+    for (int k = 0; k < MS; k++) at[k] = 0.0, bt[k] = 1.0, ct[k] = 0.0;
+    delt[0] = 0.0;
+    fluxh[0] = 0.0;
+    fluxh[MS] = 0.0;
+#pragma unroll
+    for (int k = 1; k <= MS - 1; k++) {
+      double wbl_k = MAX(0.01, wb[k - 1] - wbice[k - 1]);
+      double wbl_kp = MAX(0.01, wb[k] - wbice[k]);
+      delt[k] = wbl_kp - wbl_k;
+      double wh = MIN(wbl_k, wbl_kp);
+      if (wbice[k - 1] > 0.05 || wbice[k] > 0.01) wh = 0.9 * wbl_k + 0.1 * wbl_kp;
+      double speed_k = hyds * pow(wh / ssat, e1);
+      double rat = delt[k - 1] / (delt[k] + copysign(1.0e-20, delt[k]));
+      double phi = MAX(MAX(0.0, MIN(1.0, 2.0 * rat)), MIN(2.0, rat));
+      double fluxhi = wh, fluxlo = wbl_k;
+      speed_k = MIN(speed_k, 0.5 * zse[k - 1] / dels);
+      fluxh[k] = speed_k * (fluxlo + phi * (fluxhi - fluxlo));
+    }
+    if (wb[MS - 1] > (double)PV(soil_sfc, 0)) {
+      double wbl_k = MAX(0.001, wb[MS - 1] - wbice[MS - 1]);
+      double wbl_kp = MAX(0.001, ssat - wbice[MS - 1]);
+      double wh = MIN(wbl_k, wbl_kp);
+      if (wbice[MS - 1] > 0.05) wh = 0.9 * wbl_k + 0.1 * wbl_kp;
+      double speed_k = hyds * pow(wh / ssat, e1);
+      speed_k = 0.5 * speed_k / (1. - MIN(0.5, 10. * wbice[MS - 1]));
+      double fluxlo = wbl_k;
+      speed_k = MIN(0.5 * speed_k, 0.5 * zse[MS - 1] / dels);
+      fluxh[MS] = MAX(0.0, speed_k * fluxlo);
+    }
+#pragma unroll
+    for (int k = MS; k >= 1; k--) {
+      fluxh[k - 1] = MIN(fluxh[k - 1], (ssat - wb[k - 1]) * zse[k - 1] / dels + fluxh[k]);
+      wb[k - 1] = wb[k - 1] + dels * (fluxh[k - 1] - fluxh[k]) / zse[k - 1];
+      double ssatcurr_k = ssat - wbice[k - 1];
+      dtt[k] = dels / (zse[k - 1] * ssatcurr_k);
+      wblf[k - 1] = (wb[k - 1] - wbice[k - 1]) / ssatcurr_k;
+    }
+    t.ssnow_rnof2[0] = (R)(dels * fluxh[MS] * 1000.0);
+    double hsbh = PV(soil_hsbh, 0), pwb_min = PV(soil_pwb_min, 0);
+    double e2 = (double)PV(soil_ibp2, 0) - 1;
+#pragma unroll
+    for (int k = 2; k <= MS; k++) {
+      double wbh_k = (zse[k - 1] * wblf[k - 2] + zse[k - 2] * wblf[k - 1]) / (zse[k - 1] + zse[k - 2]);
+      double fact = pow(wbh_k, e2);
+      double pwb_wbh =
+          (hsbh * (1. - MIN(2. * MIN(0.1, MAX(wbice[k - 2] / MAX(0.01, wb[k - 2]), wbice[k - 1] / MAX(0.01, wb[k - 1]))),
+                            0.1))) *
+          MAX(pwb_min, wbh_k * fact);
+      double z3_k = pwb_wbh / a.zshh[k - 1];
+      at[k - 1] = -dtt[k] * z3_k;
+      ct[k - 2] = -dtt[k - 1] * z3_k;
+    }
+#pragma unroll
+    for (int k = 0; k < MS; k++) bt[k] = 1. - at[k] - ct[k];
+    wblf[0] = wblf[0] + dtt[1] * (double)t.ssnow_fwtop1[0] / 1000.0;
+    wblf[1] = wblf[1] + dtt[2] * (double)t.ssnow_fwtop2[0] / 1000.0;
+    wblf[2] = wblf[2] + dtt[3] * (double)t.ssnow_fwtop3[0] / 1000.0;
+    trimb<MS>(at, bt, ct, wblf);
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+      double ssatcurr = ssat - wbice[k];
+      wb[k] = wblf[k] * ssatcurr + wbice[k];
+    }
+    const double FL = (double)FROZEN_LIMIT;
+    double dfactor = (double)(K(1.0) - DENSITY_ICE / DENSITY_LIQ);
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+      if (wbice[k] > FL * wb[k]) {
+        double sicemelt = (double)(R)((wbice[k] - FL * wb[k]) / (1.0 - FL * dfactor));
+        wbice[k] = wbice[k] - sicemelt;
+        wb[k] = wb[k] - dfactor * sicemelt;
+        t.ssnow_tgg[k] = (R)((double)t.ssnow_tgg[k] - sicemelt * zse[k] * 921.0 * (double)HLF / (t.ssnow_gammzz[k]));
+      }
+    }
+  }
+
+  // src/science/soilsnow/cbl_surfbv.F90:9-146 (nglacier = 2)
+  CBM_HD void surfbv() {
+    const double *zse = a.zse;
+    R dels = (R)a.dels;
+    smoisturev();
+    double ssat = PV(soil_ssat, 0);
+    double rnof1 = (double)t.ssnow_rnof1[0];
+    double wfloor = (double)(PV(soil_swilt, 0) / (K(2.) * K(2.0)));
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+      rnof1 = (double)(R)(rnof1 + (double)(R)(MAX(t.ssnow_wb[k] - ssat, 0.0) * 1000.0) * zse[k]);
+      t.ssnow_wb[k] = MAX(wfloor, MIN(t.ssnow_wb[k], ssat));
+    }
+    t.ssnow_rnof1[0] = (R)rnof1;
+    R rnof5 = K(0.);
+    R smelt1[4] = {K(0.), K(0.), K(0.), K(0.)};
+    R smasstot = K(0.0);
+    int isflag = t.ssnow_isflag[0];
+    R *smass = t.ssnow_smass;
+    if (t.ssnow_snowd[0] > MAX_GLACIER_SNOWD) {
+      rnof5 = MIN(K(0.1), t.ssnow_snowd[0] - MAX_GLACIER_SNOWD);
+      if (isflag == 0) {
+        smasstot = K(0.0);
+        t.ssnow_tgg[0] = t.ssnow_tgg[0] - rnof5 * HLF / (R)(t.ssnow_gammzz[0]);
+        t.ssnow_snowd[0] = t.ssnow_snowd[0] - rnof5;
+      } else {
+        smasstot = smass[0] + smass[1] + smass[2];
+      }
+    }
+#pragma unroll
+    for (int k = 1; k <= 3; k++) {
+      if (t.ssnow_snowd[0] > MAX_GLACIER_SNOWD && isflag > 0) {
+        smelt1[k] = MIN(rnof5 * smass[k - 1] / smasstot, K(0.2) * smass[k - 1]);
+        smass[k - 1] = smass[k - 1] - smelt1[k];
+        t.ssnow_snowd[0] = t.ssnow_snowd[0] - smelt1[k];
+      }
+    }
+    if (isflag > 0) rnof5 = smelt1[1] + smelt1[2] + smelt1[3];
+    t.ssnow_sinfil[0] = K(0.0);
+    if (t.veg_iveg[0] == LAKES) {
+      R wbl = t.ssnow_wb_lake[0], si;
+      R sfc = PV(soil_sfc, 0), swilt = PV(soil_swilt, 0);
+      si = MIN(t.ssnow_rnof1[0], wbl);
+      t.ssnow_rnof1[0] = MAX(K(0.0), t.ssnow_rnof1[0] - si);
+      wbl = MAX(K(0.0), wbl - si);
+      si = MIN(t.ssnow_rnof2[0], wbl);
+      t.ssnow_rnof2[0] = MAX(K(0.0), t.ssnow_rnof2[0] - si);
+      wbl = MAX(K(0.0), wbl - si);
+      double z6 = zse[MS - 1];
+      double xxx = MAX(0.0, (t.ssnow_wb[MS - 1] - (double)sfc) * z6 * 1000.0);
+      si = MIN((R)xxx, wbl);
+      t.ssnow_wb[MS - 1] = t.ssnow_wb[MS - 1] - (double)si / (z6 * 1000.0);
+      wbl = MAX(K(0.0), wbl - si);
+      xxx = MAX(0.0, (t.ssnow_wb[MS - 1] - (double)(K(0.5) * (sfc + swilt))) * z6 * 1000.0);
+      si = MIN((R)xxx, wbl);
+      t.ssnow_wb[MS - 1] = t.ssnow_wb[MS - 1] - (double)si / (z6 * 1000.0);
+      wbl = MAX(K(0.0), wbl - si);
+      t.ssnow_wb_lake[0] = wbl;
+      t.ssnow_sinfil[0] = si;
+    }
+    t.ssnow_rnof1[0] = t.ssnow_rnof1[0] / dels + rnof5 / dels;
+    t.ssnow_rnof2[0] = t.ssnow_rnof2[0] / dels;
+    t.ssnow_runoff[0] = t.ssnow_rnof1[0] + t.ssnow_rnof2[0];
+  }
+
+  // src/science/soilsnow/cbl_soilsnow_main.F90:28-207
+  CBM_HD void soil_snow() {
+    const double *zse = a.zse;
+    R dels = (R)a.dels;
+    if constexpr (FULL) {
+      R zsetot = (R)zse[0];
+#pragma unroll
+      for (int k = 1; k < MS; k++) zsetot = zsetot + (R)zse[k];
+      R tg = K(0.);
+#pragma unroll
+      for (int k = 0; k < MS; k++) tg = tg + (((R)zse[k] / zsetot) * t.ssnow_tgg[k]);
+      t.ssnow_tggav[0] = tg;
+    }
+    R css = PV(soil_css, 0), rhosoil = PV(soil_rhosoil, 0), ssat = PV(soil_ssat, 0);
+    R hcll = MAX(K(0.01), css * rhosoil);
+#pragma unroll
+    for (int k = 0; k < MS; k++) t.soil_heat_cap_lower_limit[k] = hcll;
+    t.ssnow_t_snwlr[0] = K(0.05);
+    t.ssnow_fwtop1[0] = t.ssnow_fwtop2[0] = t.ssnow_fwtop3[0] = K(0.0);
+    t.ssnow_runoff[0] = K(0.0);
+    t.ssnow_rnof1[0] = K(0.0);
+    t.ssnow_rnof2[0] = K(0.0);
+    t.ssnow_smelt[0] = K(0.0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) t.ssnow_dtmlt[k] = K(0.0);
+    t.ssnow_osnowd[0] = t.ssnow_snowd[0];
+#pragma unroll
+    for (int k = 0; k < MS; k++) t.ssnow_wbliq[k] = t.ssnow_wb[k] - t.ssnow_wbice[k];
+    if (a.first_call) {
+      double xx = (double)(css * rhosoil);
+      t.ssnow_gammzz[0] = MAX((double)((K(1.0) - ssat) * css * rhosoil) +
+                                  (t.ssnow_wb[0] - t.ssnow_wbice[0]) * (double)CSWAT * (double)DENSITY_LIQ +
+                                  t.ssnow_wbice[0] * (double)CSICE * (double)DENSITY_ICE,
+                              xx) * zse[0] +
+                          (double)((K(1.) - (R)t.ssnow_isflag[0]) * CGSNOW * t.ssnow_snowd[0]);
+    }
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+      t.ssnow_wblf[k] = MAX(0.01, (t.ssnow_wb[k] - t.ssnow_wbice[k])) / (double)ssat;
+      t.ssnow_wbfice[k] = (double)(R)(t.ssnow_wbice[k] / (double)ssat);
+    }
+    snowcheck();
+    snowdensity();
+    snow_accum();
+    R snowmlt = snow_melting();
+    t.ssnow_smelt[0] = snowmlt;
+    snowl_adjust();
+    stempv();
+    int isflag = t.ssnow_isflag[0];
+    t.ssnow_tss[0] = (R)(1 - isflag) * t.ssnow_tgg[0] + (R)isflag * t.ssnow_tggsn[0];
+    snowmlt = snow_melting();
+    t.ssnow_smelt[0] = t.ssnow_smelt[0] + snowmlt;
+    remove_trans();
+    soilfreeze();
+    R totwet = t.canopy_precis[0] + t.ssnow_smelt[0];
+    double weting = (double)totwet + MAX(0.0, (double)t.ssnow_pudsto[0] - t.canopy_fesp[0] / (double)HL * (double)dels);
+    double dssat = (double)ssat;
+    double xxx = dssat - t.ssnow_wb[0];
+    double sinfil1 = MIN(0.95 * xxx * zse[0] * 1000.0, weting);
+    xxx = dssat - t.ssnow_wb[1];
+    double sinfil2 = MIN(0.95 * xxx * zse[1] * 1000.0, weting - sinfil1);
+    xxx = dssat - t.ssnow_wb[2];
+    double sinfil3 = MIN(0.95 * xxx * zse[2] * 1000.0, weting - sinfil1 - sinfil2);
+    t.ssnow_fwtop1[0] = (R)(sinfil1 / (double)dels - (double)t.canopy_segg[0]);
+    t.ssnow_fwtop2[0] = (R)(sinfil2 / (double)dels);
+    t.ssnow_fwtop3[0] = (R)(sinfil3 / (double)dels);
+    R pudsto = (R)MAX(0.0, weting - sinfil1 - sinfil2 - sinfil3);
+    t.ssnow_rnof1[0] = MAX(K(0.), pudsto - (R)PV(ssnow_pudsmx, 0));
+    t.ssnow_pudsto[0] = pudsto - t.ssnow_rnof1[0];
+    surfbv();
+    t.ssnow_smelt[0] = t.ssnow_smelt[0] / dels;
+    isflag = t.ssnow_isflag[0];
+    t.ssnow_tss[0] = (R)(1 - isflag) * t.ssnow_tgg[0] + (R)isflag * t.ssnow_tggsn[0];
+#pragma unroll
+    for (int k = 0; k < MS; k++) t.ssnow_wbliq[k] = t.ssnow_wb[k] - t.ssnow_wbice[k];
+    if constexpr (FULL) t.ssnow_totsdepth[0] = t.ssnow_sdepth[0] + t.ssnow_sdepth[1] + t.ssnow_sdepth[2];
+    double wbtot = 0.0;
+#pragma unroll
+    for (int k = 0; k < MS; k++) wbtot = wbtot + (t.ssnow_wbliq[k] * 1000.0 + t.ssnow_wbice[k] * 921.0) * zse[k];
+    t.ssnow_wbtot[0] = wbtot;
+  }
+
+  // src/science/soilsnow/cbl_snow_aging.F90:11-83
+  CBM_HD void snow_aging() {
+    R snowd = t.ssnow_snowd[0];
+    if (snowd > K(1.0)) {
+      R dnsnow = MIN(K(1.0), K(0.1) * MAX(K(0.0), snowd - t.ssnow_osnowd[0]));
+      int fl = t.ssnow_isflag[0];
+      R tmp = (R)fl * t.ssnow_tggsn[0] + (R)(1 - fl) * t.ssnow_tgg[0];
+      tmp = MIN(tmp, TFRZ);
+      R ar1 = K(5000.0) * (K(1.0) / (TFRZ - K(0.01)) - K(1.0) / tmp);
+      R ar2 = K(10.0) * ar1;
+      R ar3;
+      if (t.soil_isoilm[0] == 9) {
+        ar3 = K(0.0000001);
+        dnsnow = K(1.0);
+      } else {
+        ar3 = K(0.1);
+      }
+      R dtau = K(1.0e-6) * (exp(ar1) + exp(ar2) + ar3) * (R)a.dels;
+      t.ssnow_snage[0] = MAX(K(0.0), (t.ssnow_snage[0] + dtau) * (K(1.0) - dnsnow));
+    }
+  }
+
+  // src/science/misc/cable_carbon.F90:38-360 (mvtype 17, DIAG_SOIL_RESP on)
+  CBM_HD static R rw17(int i) {
+    const R v[17] = {K(16.), K(16.), K(18.), K(8.7), K(12.5), K(15.1), K(10.4), K(7.5), K(6.1),
+                     K(6.1), K(0.001), K(5.8), K(0.001), K(5.8), K(0.001), K(9.0), K(0.001)};
+    return v[i];
+  }
+  CBM_HD static R tfcl17(int i) {
+    const R v[17] = {K(0.42), K(0.248), K(0.38), K(0.345), K(0.31), K(0.73), K(2.4), K(0.35), K(0.997),
+                     K(0.997), K(0.9500), K(0.55), K(0.9500), K(0.55), K(0.9500), K(2.4), K(0.9500)};
+    return v[i];
+  }
+  CBM_HD static R tvclst17(int i) {
+    const R v[17] = {K(235.), K(283.), K(268.), K(278.), K(278.), K(235.), K(278.0), K(278.0),
+                     K(278.0), K(278.0), K(278.0), K(278.), K(278.), K(278.), K(268.), K(278.), K(278.)};
+    return v[i];
+  }
+  CBM_HD void carbon() {
+    R dels = (R)a.dels;
+    R rc0 = (R)a.ratecp[0], rc1 = (R)a.ratecp[1], rc2 = (R)a.ratecp[2];
+    R *cp = t.bgc_cplant, *cs = t.bgc_csoil;
+    R S = rc0 * cp[0] + rc1 * cp[1] + rc2 * cp[2];
+    R pc1 = (S - rc0 * cp[0]);
+    R pc1w = (S - rc0 * cp[0] - rc2 * cp[2]);
+    R pc1r = (S - rc0 * cp[0] - rc1 * cp[1]);
+    R tk = t.met_tk[0];
+    R tmp1 = MAX(K(3.22) - K(0.046) * (tk - TFRZ), K(1e-6));
+    R tmp2 = K(0.1) * (tk - TFRZ - K(20.0));
+    R tmp3 = pow(tmp1, tmp2);
+    const R spy = K(365.0) * K(24.0) * K(3600.0);
+    R rp20 = PV(veg_rp20, 0);
+    R frp = rp20 * tmp3 * pc1 / spy;
+    R frpw = rp20 * tmp3 * pc1w / spy;
+    R frpr = rp20 * tmp3 * pc1r / spy;
+    t.canopy_frp[0] = frp;
+    t.canopy_frpw[0] = frpw;
+    t.canopy_frpr[0] = frpr;
+    R swilt = PV(soil_swilt, 0);
+    {
+      const R t0 = K(-46.0), rswch = K(0.16), soilcf = K(1.0);
+      R den = MAX(K(0.07), PV(soil_sfc, 0) - swilt);
+      R rswc = MAX(K(0.0001), PV(veg_froot, 0) * ((R)t.ssnow_wb[1] - swilt)) / den;
+      R tsoil = PV(veg_froot, 0) * t.ssnow_tgg[1] - TFRZ;
+      R tref = MAX(K(0.), t.ssnow_tgg[MS - 1] - (TFRZ - K(.05)));
+#pragma unroll
+      for (int k = 1; k < MS; k++) {
+        rswc = rswc + MAX(K(0.0001), PV(veg_froot, k) * ((R)t.ssnow_wb[k] - swilt)) / den;
+        tsoil = tsoil + PV(veg_froot, k) * t.ssnow_tgg[k];
+      }
+      rswc = MIN(K(1.), rswc);
+      tsoil = MAX(t0 + K(2.), tsoil);
+      R e0rswc = K(52.4) + K(285.) * rswc;
+      R ftsoil = MIN(K(0.0015), K(1.) / (tref - t0) - K(1.) / (tsoil - t0));
+      R sss = MAX(K(-15.), MIN(K(1.), e0rswc * ftsoil));
+      R ftsrs = exp(sss);
+      t.canopy_frs[0] = PV(veg_vegcf, 0) * (K(144.0) / K(44.0e6)) * soilcf * MIN(K(1.), K(1.4) * MAX(K(.3), K(.0278) * tsoil + K(.5))) *
+                        ftsrs * rswc / (rswch + rswc);
+    }
+    {
+      const R beta = K(0.9), trnl = K(3.17e-8), trnr = K(4.53e-9), trnsf = K(1.057e-10), trnw = K(6.342e-10);
+      int iv = t.veg_iveg[0] - 1;
+      R coef_cold = exp(MIN(K(1.), -(t.canopy_tv[0] - tvclst17(iv))));
+      R wbav = PV(veg_froot, 0) * (R)t.ssnow_wb[0];
+#pragma unroll
+      for (int k = 1; k < MS; k++) wbav = wbav + PV(veg_froot, k) * (R)t.ssnow_wb[k];
+      wbav = MAX(K(0.01), wbav);
+      R ce = K(2.0) - PV(soil_ibp2, 0);
+      R esw = pow(wbav, ce) - K(1.0);
+      esw = MAX(K(1.0), esw);
+      R eswilt = pow(swilt, ce) - K(1.0);
+      R rel = esw / eswilt - K(1.0);
+      rel = MIN(K(1.0), rel);
+      R coef_drght = exp(K(5.0) * rel);
+      R coef_cd = (coef_cold + coef_drght) * K(2.0e-7);
+      R fcl = exp(-tfcl17(iv) * t.veg_vlai[0]);
+      R clitt = (coef_cd + trnl) * cp[0];
+      R fpn = t.canopy_fpn[0];
+      cp[0] = cp[0] - dels * (fpn * fcl + clitt);
+      R fr = MIN(K(1.), exp(-rw17(iv) * beta * K(0.0001) * cp[2] / MAX(cp[1], K(0.01))) / beta);
+      R cfwd = trnw * cp[1];
+      cp[1] = cp[1] - dels * (fpn * (K(1.) - fcl) * (K(1.) - fr) + frpw + cfwd);
+      R cfrts = trnr * cp[2];
+      cp[2] = cp[2] - dels * (fpn * (K(1.) - fcl) * fr + cfrts + frpr);
+      R cfsf = trnsf * cs[0];
+      R frs = t.canopy_frs[0];
+      cs[0] = cs[0] + dels * (K(0.98) * clitt + K(0.9) * cfrts + cfwd - cfsf - K(0.98) * frs);
+      cs[1] = cs[1] + dels * (K(0.02) * clitt + K(0.1) * cfrts + cfsf - K(0.02) * frs);
+#pragma unroll
+      for (int k = 0; k < 3; k++) cp[k] = MAX(K(0.00), cp[k]);
+#pragma unroll
+      for (int k = 0; k < 2; k++) cs[k] = MAX(K(0.00), cs[k]);
+    }
+  }
+
+  // ===================================================================
+  // src/offline/cbl_model_driver_offline.F90:38-231, with the offline
+  // driver's contract (src/offline/cable_serial.F90:572-575, 602-605,
+  // 725-729) and the balance accumulators (src/offline/cable_checks.F90
+  // :472-618) in FP64
+  CBM_HD void run() {
+    R dels = (R)a.dels;
+    t.met_ofsd[0] = t.met_fsd[0] + t.met_fsd[1];
+    t.canopy_oldcansto[0] = t.canopy_cansto[0];
+    if (t.veg_iveg[0] >= 14) t.veg_vlai[0] = K(0.);
+    if (t.veg_iveg[0] == LAKES && t.ssnow_wb[0] < (double)PV(soil_sfc, 0)) {
+      t.ssnow_wbtot1[0] = (R)((t.ssnow_wb[0]) * 1000.0 * a.zse[0]);
+      t.ssnow_wb[0] = (double)PV(soil_sfc, 0);
+      t.ssnow_wbtot2[0] = (R)((t.ssnow_wb[0]) * 1000.0 * a.zse[0]);
+    }
+    t.ssnow_wb_lake[0] = t.ssnow_wb_lake[0] + MAX(t.ssnow_wbtot2[0] - t.ssnow_wbtot1[0], K(0.));
+    ruff_resist();
+    define_air();
+    bool veg_mask = t.canopy_vlaiw[0] > LAI_THRESH;
+    bool sunlit = (t.met_fsd[0] + t.met_fsd[1]) > RAD_THRESH;
+    bool sunlit_veg = veg_mask && sunlit;
+    init_radiation_albedo(veg_mask);
+    t.rad_albedo_t[0] = (t.rad_albedo[0] + t.rad_albedo[1]) * K(0.5);
+    t.ssnow_otss_0[0] = t.ssnow_otss[0];
+    t.ssnow_otss[0] = t.ssnow_tss[0];
+    define_canopy(sunlit_veg);
+    t.ssnow_owetfac[0] = t.ssnow_wetfac[0];
+    soil_snow();
+    snow_aging();
+    t.ssnow_deltss[0] = t.ssnow_tss[0] - t.ssnow_otss[0];
+    t.canopy_fev[0] = (R)(t.canopy_fevc[0] + (double)t.canopy_fevw[0]);
+    t.canopy_fe[0] = (R)((double)t.canopy_fev[0] + t.canopy_fes[0]);
+    t.canopy_rnet[0] = t.canopy_fns[0] + t.canopy_fnv[0];
+    R transd = t.rad_transd[0];
+    t.rad_trad[0] = pow((K(1.) - transd) * p4(t.canopy_tv[0]) + transd * p4(t.ssnow_tss[0]), K(0.25));
+    carbon();
+    R fpn = t.canopy_fpn[0];
+    t.canopy_fnpp[0] = -K(1.0) * fpn - t.canopy_frp[0];
+    t.canopy_fgpp[0] = -K(1.0) * fpn + t.canopy_frday[0];
+    t.canopy_fnee[0] = fpn + t.canopy_frs[0] + t.canopy_frp[0];
+    t.canopy_fra[0] = t.canopy_frp[0] + t.canopy_frday[0];
+    t.ssnow_smelt[0] = t.ssnow_smelt[0] * dels;
+    t.ssnow_rnof1[0] = t.ssnow_rnof1[0] * dels;
+    t.ssnow_rnof2[0] = t.ssnow_rnof2[0] * dels;
+    t.ssnow_runoff[0] = t.ssnow_runoff[0] * dels;
+    // mass_balance, src/offline/cable_checks.F90:472-551 (FP64 accumulators)
+    {
+      if (a.ktau == 1) t.drv_owb[0] = t.ssnow_wbtot[0];
+      double delwb = t.ssnow_wbtot[0] - t.drv_owb[0];
+      t.drv_owb[0] = t.ssnow_wbtot[0];
+      double rlam = (double)t.air_rlam[0], cls = (double)t.ssnow_cls[0], dd = (double)dels;
+      double wbal = (double)t.met_precip[0] - (double)t.canopy_delwc[0] - (double)t.ssnow_snowd[0] +
+                    (double)t.ssnow_osnowd[0] - (double)t.ssnow_runoff[0] -
+                    ((double)t.canopy_fevw[0] + t.canopy_fevc[0] + t.canopy_fes[0] / cls) * dd / rlam - delwb -
+                    PV(ssnow_qrecharge, 0);
+      t.bal_wbal[0] = (R)wbal;
+      if (a.ktau == 1) {
+        t.bal_wbal_tot[0] = 0.;
+        t.bal_precip_tot[0] = 0.;
+        t.bal_rnoff_tot[0] = 0.;
+        t.bal_evap_tot[0] = 0.;
+      }
+      if (a.ktau > 10) {
+        t.bal_wbal_tot[0] = t.bal_wbal_tot[0] + (double)t.bal_wbal[0];
+        t.bal_precip_tot[0] = t.bal_precip_tot[0] + (double)t.met_precip[0];
+        t.bal_rnoff_tot[0] = t.bal_rnoff_tot[0] + (double)t.ssnow_rnof1[0] + (double)t.ssnow_rnof2[0];
+        t.bal_evap_tot[0] = t.bal_evap_tot[0] + ((double)t.canopy_fev[0] + t.canopy_fes[0] / cls) * dd / rlam;
+      }
+    }
+    // energy_balance, src/offline/cable_checks.F90:565-618
+    {
+      R *fsd = t.met_fsd, *alb = t.rad_albedo, *qc = t.rad_qcan;
+      R tv4 = p4(t.canopy_tv[0]);
+      R radbal = fsd[0] + fsd[1] + t.met_fld[0] - alb[0] * fsd[0] - alb[1] * fsd[1] -
+                 (EMSOIL * SBOLTZ * transd * p4(t.ssnow_otss[0])) - (EMLEAF * SBOLTZ * (K(1) - transd) * tv4) -
+                 t.canopy_fnv[0] - t.canopy_fns[0];
+      R fes = (R)t.canopy_fes[0];
+      R ebal = (qc[0] + qc[1]) + (qc[2] + qc[3]) + t.rad_qssabs[0] + t.met_fld[0] - SBOLTZ * EMLEAF * tv4 * (K(1) - transd) -
+               t.rad_flws[0] * transd - t.canopy_fev[0] - fes - t.canopy_fh[0] - t.canopy_ga[0];
+      t.bal_radbal[0] = radbal;
+      t.bal_ebal[0] = ebal;
+      if constexpr (FULL) {
+        t.bal_ebalsoil[0] = t.canopy_fns[0] - fes - t.canopy_fhs[0] - t.canopy_ga[0];
+        t.bal_ebalveg[0] = t.canopy_fnv[0] - t.canopy_fev[0] - t.canopy_fhv[0];
+      }
+      t.bal_ebal_tot[0] = t.bal_ebal_tot[0] + (double)ebal;
+      t.bal_radbalsum[0] = t.bal_radbalsum[0] + (double)radbal;
+    }
+  }
+};
+
+}  // namespace cbm

@@ -21,7 +21,7 @@ class Field:
     fid: int
     ident: str
     name: str      # Fortran component name, e.g. "ssnow%wb"
-    prec: str      # "F" default REAL, "D" REAL(r_2), "I" INTEGER
+    prec: str      # "F" default REAL, "D" REAL(r_2), "I" INTEGER, "B" FP64 accumulator
     n: int         # elements per tile
     role: str      # PARAM, FORC, STATE, DIAG
 
@@ -36,7 +36,7 @@ def _parse():
     txt = open(HEADER).read()
     body = txt[txt.index("#define CABLE_FIELDS(X)"):txt.index("enum cable_field_id")]
     out = []
-    for m in re.finditer(r'X\((\w+),\s*"([^"]+)",\s*([FDI]),\s*(\d+),\s*(\w+)\)', body):
+    for m in re.finditer(r'X\((\w+),\s*"([^"]+)",\s*([FDIB]),\s*(\d+),\s*(\w+)\)', body):
         out.append(Field(len(out), m.group(1), m.group(2), m.group(3), int(m.group(4)), m.group(5)))
     return out
 

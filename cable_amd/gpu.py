@@ -1,0 +1,224 @@
+"""Python host side of the MI355X CABLE timestep (ctypes over libcable_gpu.so).
+
+The product path: every call goes to the HIP library; when the library is
+missing the import fails loudly (there is no CPU fallback).
+
+``CableGPU`` mirrors the reference's step interface: the caller owns the
+derived-type fields (here {Fortran name: numpy array}), attaches them, and
+calls ``cbm(ktau, dels)`` once per step exactly as the offline driver calls
+``cbm`` (src/offline/cable_serial.F90:594); ``run`` is the device-resident
+multi-step path used for production runs and the benchmark.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .fields import BY_NAME, FIELDS, NFIELDS
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libcable_gpu.so")
+
+PREC_FP64, PREC_MIXED = 0, 1
+HOST_F64, HOST_F32, HOST_I32 = 0, 1, 2
+
+EXPORTS = [
+    "cable_gpu_nfields", "cable_gpu_field_info", "cable_gpu_field_id",
+    "cable_gpu_create", "cable_gpu_destroy", "cable_gpu_upload",
+    "cable_gpu_download", "cable_gpu_set_globals", "cable_gpu_step",
+    "cable_gpu_veg_cols", "cable_gpu_soil_cols", "cable_gpu_table_column",
+    "cable_gpu_set_class_tables", "cable_gpu_set_forcing", "cable_gpu_run",
+    "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters",
+    "cable_gpu_last_error",
+]
+
+_lib = None
+
+
+class CableError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libcable_gpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise CableError(f"{LIBPATH} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIBPATH)
+    vp, i64, c_int, c_double = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    sig = {
+        "cable_gpu_nfields": ([], c_int),
+        "cable_gpu_field_info": ([c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p,
+                                  ctypes.POINTER(c_int), ctypes.POINTER(c_int)], c_int),
+        "cable_gpu_field_id": ([ctypes.c_char_p], c_int),
+        "cable_gpu_create": ([ctypes.POINTER(vp), i64, c_int], c_int),
+        "cable_gpu_destroy": ([vp], c_int),
+        "cable_gpu_upload": ([vp, c_int, vp, c_int], c_int),
+        "cable_gpu_download": ([vp, c_int, vp, c_int], c_int),
+        "cable_gpu_set_globals": ([vp, vp, vp, vp, vp], c_int),
+        "cable_gpu_step": ([vp, c_int, c_double], c_int),
+        "cable_gpu_veg_cols": ([], c_int),
+        "cable_gpu_soil_cols": ([], c_int),
+        "cable_gpu_table_column": ([c_int, c_int], c_int),
+        "cable_gpu_set_class_tables": ([vp, vp, vp], c_int),
+        "cable_gpu_set_forcing": ([vp, c_int] + [vp] * 8 + [c_int, c_double, c_double], c_int),
+        "cable_gpu_run": ([vp, c_int, c_int, c_double], c_int),
+        "cable_gpu_sync": ([vp], c_int),
+        "cable_gpu_last_run_ms": ([vp, ctypes.POINTER(c_double)], c_int),
+        "cable_gpu_counters": ([vp, ctypes.POINTER(i64)], c_int),
+        "cable_gpu_last_error": ([], ctypes.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    if L.cable_gpu_nfields() != NFIELDS:
+        raise CableError("libcable_gpu.so was built from a different field table")
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise CableError(lib().cable_gpu_last_error().decode())
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def class_tables(vtab: dict, stab: dict):
+    """Pack the workload's per-class tables into the library's column layout."""
+    L = lib()
+    nv, ns = L.cable_gpu_veg_cols(), L.cable_gpu_soil_cols()
+    V = np.zeros((17, nv))
+    S = np.zeros((9, ns))
+    vmap = {"veg%canst1": "canst1", "veg%dleaf": "dleaf", "veg%ejmax": "ejmax", "veg%frac4": "frac4",
+            "veg%hc": "hc", "veg%rp20": "rp20", "veg%shelrb": "shelrb", "veg%vegcf": "vegcf",
+            "veg%vbeta": "vbeta", "veg%vcmax": "vcmax", "veg%xfang": "xfang", "veg%extkn": "extkn",
+            "veg%a1gs": "a1gs", "veg%d0gs": "d0gs", "veg%alpha": "alpha", "veg%convex": "convex",
+            "veg%cfrd": "cfrd", "veg%gswmin": "gswmin", "veg%conkc0": "conkc0", "veg%conko0": "conko0",
+            "veg%ekc": "ekc", "veg%eko": "eko"}
+    for fname, key in vmap.items():
+        V[:, L.cable_gpu_table_column(0, BY_NAME[fname].fid)] = vtab[key]
+    V[:, L.cable_gpu_table_column(0, BY_NAME["veg%meth"].fid)] = 1.0
+    c = L.cable_gpu_table_column(0, BY_NAME["veg%refl"].fid)
+    V[:, c:c + 2] = vtab["refl"][:2].T
+    c = L.cable_gpu_table_column(0, BY_NAME["veg%taul"].fid)
+    V[:, c:c + 2] = vtab["taul"][:2].T
+    c = L.cable_gpu_table_column(0, BY_NAME["veg%froot"].fid)
+    V[:, c:c + 6] = vtab["froot"].T
+    V[:, L.cable_gpu_table_column(0, BY_NAME["rough%za_uv"].fid)] = vtab.get("za_uv", 40.0)
+    V[:, L.cable_gpu_table_column(0, BY_NAME["rough%za_tq"].fid)] = vtab.get("za_tq", 40.0)
+    for key in ("css", "hsbh", "hyds", "i2bp3", "ibp2", "rhosoil", "sfc", "ssat", "swilt", "cnsd", "pwb_min"):
+        S[:, L.cable_gpu_table_column(1, BY_NAME[f"soil%{key}"].fid)] = stab[key]
+    for key in ("sfc", "swilt"):
+        c = L.cable_gpu_table_column(1, BY_NAME[f"soil%{key}_vec"].fid)
+        S[:, c:c + 6] = np.tile(stab[key][:, None], (1, 6))
+    c = L.cable_gpu_table_column(1, BY_NAME["soil%zse_vec"].fid)
+    S[:, c:c + 6] = np.array(stab["zse"])[None, :]
+    return np.ascontiguousarray(V), np.ascontiguousarray(S)
+
+
+class CableGPU:
+    """Device-resident state of ``mp`` tiles."""
+
+    def __init__(self, mp: int, precision: int = PREC_FP64):
+        self.mp = int(mp)
+        self.precision = precision
+        h = ctypes.c_void_p()
+        _check(lib().cable_gpu_create(ctypes.byref(h), self.mp, int(precision)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cable_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- fields ------------------------------------------------------------
+    def upload(self, name: str, arr):
+        f = BY_NAME[name]
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        if a.size != f.n * self.mp:
+            raise CableError(f"{name}: {a.size} elements, expected {f.n * self.mp}")
+        _check(lib().cable_gpu_upload(self.h, f.fid, _ptr(a), HOST_F64))
+
+    def download(self, name: str):
+        f = BY_NAME[name]
+        out = np.empty(f.shape(self.mp), dtype=np.float64)
+        _check(lib().cable_gpu_download(self.h, f.fid, _ptr(out), HOST_F64))
+        return out
+
+    def upload_all(self, fields: dict):
+        for k, v in fields.items():
+            if k in BY_NAME:
+                self.upload(k, v)
+
+    def set_globals(self, zse, zshh, ratecp, ratecs):
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (zse, zshh, ratecp, ratecs)]
+        self._globals = arrs
+        _check(lib().cable_gpu_set_globals(self.h, *[_ptr(a) for a in arrs]))
+
+    # -- one step, every diagnostic (the cbm drop-in) ------------------------
+    def cbm(self, ktau: int, dels: float):
+        _check(lib().cable_gpu_step(self.h, int(ktau), float(dels)))
+
+    def dryleaf_passes(self):
+        v = ctypes.c_int64()
+        _check(lib().cable_gpu_counters(self.h, ctypes.byref(v)))
+        return v.value
+
+    # -- device-resident run -----------------------------------------------
+    def set_class_tables(self, V, S):
+        self._tabs = (np.ascontiguousarray(V, dtype=np.float64), np.ascontiguousarray(S, dtype=np.float64))
+        _check(lib().cable_gpu_set_class_tables(self.h, _ptr(self._tabs[0]), _ptr(self._tabs[1])))
+
+    def set_forcing(self, forcing: dict, ca=400.0e-6, doy=172.0):
+        """forcing: {met%fsd (ns,2,mp) or fsd1 (ns,mp), met%fld, met%tk, met%qv, met%ua,
+        met%pmb, met%precip, met%coszen}, each (nsteps, mp)."""
+        fsd = forcing["met%fsd"]
+        fsd1 = fsd[:, 0, :] if fsd.ndim == 3 else fsd
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in
+                (fsd1, forcing["met%fld"], forcing["met%tk"], forcing["met%qv"], forcing["met%ua"],
+                 forcing["met%pmb"], forcing["met%precip"], forcing["met%coszen"])]
+        ns = arrs[0].shape[0]
+        _check(lib().cable_gpu_set_forcing(self.h, ns, *[_ptr(a) for a in arrs], HOST_F64,
+                                           float(ca), float(doy)))
+        self.nsteps = ns
+
+    def run(self, k0: int, nsteps: int, dels: float):
+        _check(lib().cable_gpu_run(self.h, int(k0), int(nsteps), float(dels)))
+
+    def last_run_ms(self):
+        v = ctypes.c_double()
+        _check(lib().cable_gpu_last_run_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def sync(self):
+        _check(lib().cable_gpu_sync(self.h))
+
+
+def from_workload(wl, precision=PREC_FP64, full=True):
+    """A CableGPU holding workload ``wl``'s parameters and initial state (every
+    other field zero, as the reference's derived types start)."""
+    g = CableGPU(wl.n, precision)
+    g.upload_all(wl.params)
+    g.upload_all(wl.state0)
+    p = wl.params
+    g.set_globals(p["soil%zse"], p["soil%zshh"], p["bgc%ratecp"], p["bgc%ratecs"])
+    if not full:
+        stab = dict(wl.stab)
+        stab["zse"] = p["soil%zse"]
+        g.set_class_tables(*class_tables(wl.vtab, stab))
+        g.set_forcing(wl.forcing)
+    return g

@@ -5,7 +5,9 @@
  * Each entry is X(ident, "fortran name", prec, n, role):
  *   fortran name  the derived-type component in the reference
  *                 (src/offline/cable_define_types.F90:79-717);
- *   prec          F = default REAL, D = REAL(r_2) (always FP64), I = INTEGER;
+ *   prec          F = default REAL, D = REAL(r_2) (always FP64), I = INTEGER,
+ *                 B = default REAL in the reference but FP64 in every GPU mode
+ *                 (the per-tile water and energy balance accumulators);
  *   n             elements per tile; the array is the Fortran (mp, ...) array,
  *                 i.e. element k of tile t sits at k*mp + t (tile fastest);
  *   role          PARAM  per-tile parameter, read only;
@@ -112,12 +114,12 @@
   X(canopy_cansto, "canopy%cansto", F, 1, STATE)                           \
   X(bgc_cplant, "bgc%cplant", F, 3, STATE)                                 \
   X(bgc_csoil, "bgc%csoil", F, 2, STATE)                                   \
-  X(bal_ebal_tot, "bal%ebal_tot", F, 1, STATE)                             \
-  X(bal_wbal_tot, "bal%wbal_tot", F, 1, STATE)                             \
-  X(bal_precip_tot, "bal%precip_tot", F, 1, STATE)                         \
-  X(bal_rnoff_tot, "bal%rnoff_tot", F, 1, STATE)                           \
-  X(bal_evap_tot, "bal%evap_tot", F, 1, STATE)                             \
-  X(bal_radbalsum, "bal%radbalsum", F, 1, STATE)                           \
+  X(bal_ebal_tot, "bal%ebal_tot", B, 1, STATE)                             \
+  X(bal_wbal_tot, "bal%wbal_tot", B, 1, STATE)                             \
+  X(bal_precip_tot, "bal%precip_tot", B, 1, STATE)                         \
+  X(bal_rnoff_tot, "bal%rnoff_tot", B, 1, STATE)                           \
+  X(bal_evap_tot, "bal%evap_tot", B, 1, STATE)                             \
+  X(bal_radbalsum, "bal%radbalsum", B, 1, STATE)                           \
   /* total soil water at the previous balance check (the SAVEd owb of     \
      src/offline/cable_checks.F90:496-513), one value per tile */          \
   X(drv_owb, "drv%owb", D, 1, STATE)                                       \

@@ -1,0 +1,131 @@
+/*
+ * cable_gpu.h -- C ABI of the MI355X CABLE timestep library (libcable_gpu.so).
+ *
+ * The library replaces one call of the reference's per-tile physics,
+ *   SUBROUTINE cbm(ktau, dels, air, bgc, canopy, met, bal, rad, rough, soil,
+ *                  ssnow, sum_flux, veg, climate, xk, c1, rhoch)
+ *   (src/offline/cbl_model_driver_offline.F90:38-231),
+ * together with the offline driver's contract around it
+ * (src/offline/cable_serial.F90:572-575 before the call, :602-605 after it)
+ * and the per-tile balance accumulators (src/offline/cable_checks.F90:472-618,
+ * called at cable_serial.F90:725-729).
+ *
+ * Data model.  Every per-tile field of include/cable_fields.h lives in device
+ * memory as the Fortran (mp, ...) array, tile index fastest.  A host array is
+ * attached by field id and copied in or out; the Fortran wrapper passes
+ * C_LOC of the derived-type component (ssnow%wb, canopy%fe, ...).  The
+ * device copy is authoritative between a step and the next download.
+ *
+ * Errors.  Every function returns 0 on success and a nonzero CABLE_E_* code
+ * otherwise; cable_gpu_last_error() gives the message of the calling
+ * thread's last failure.  The reference aborts (cable_abort -> error stop,
+ * src/util/error_handler/cable_error_handler.F90:48-61); the Fortran wrapper
+ * turns a nonzero code into that abort.
+ *
+ * Threading.  A context is used by one host thread at a time; its work is
+ * ordered on the context's own HIP stream.
+ */
+#ifndef CABLE_GPU_H
+#define CABLE_GPU_H
+
+#include <stdint.h>
+
+#include "cable_fields.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cable_gpu_ctx cable_gpu_ctx;
+
+/* precision of the reference's default-REAL quantities on the device */
+enum {
+  CABLE_PREC_FP64 = 0,  /* plain: everything in FP64 */
+  CABLE_PREC_MIXED = 1  /* FP32 radiation, roughness, canopy iteration; FP64
+                           soil-moisture and soil-temperature solves and the
+                           balance accumulators */
+};
+
+/* host element types for upload/download */
+enum { CABLE_HOST_F64 = 0, CABLE_HOST_F32 = 1, CABLE_HOST_I32 = 2 };
+
+enum {
+  CABLE_OK = 0,
+  CABLE_E_ARG = 1,      /* invalid argument (null pointer, bad id, size) */
+  CABLE_E_HIP = 2,      /* HIP runtime error (allocation, copy, launch) */
+  CABLE_E_STATE = 3,    /* call out of order (e.g. run before forcing) */
+  CABLE_E_NOSYM = 4
+};
+
+/* Field table (include/cable_fields.h).  prec: 'F','D','I','B'; role:
+   0 PARAM, 1 FORC, 2 STATE, 3 DIAG.  Returns CABLE_E_ARG for a bad id. */
+int cable_gpu_nfields(void);
+int cable_gpu_field_info(int id, const char **name, char *prec, int *n, int *role);
+/* id of a Fortran component name such as "ssnow%wb", or -1 */
+int cable_gpu_field_id(const char *name);
+
+/* Create a context for mp tiles.  Allocates the device copy of every field
+   (zero-filled, as the reference's harness zero-fills its derived types) on
+   the current HIP device.  Replaces the allocation the reference performs in
+   alloc_cbm_var (src/offline/cable_define_types.F90:725-741). */
+int cable_gpu_create(cable_gpu_ctx **ctx, int64_t mp, int precision);
+int cable_gpu_destroy(cable_gpu_ctx *ctx);
+
+/* Copy one field between a host array of mp*n elements and the device. */
+int cable_gpu_upload(cable_gpu_ctx *ctx, int field_id, const void *host, int host_type);
+int cable_gpu_download(cable_gpu_ctx *ctx, int field_id, void *host, int host_type);
+
+/* Grid-wide parameters: soil%zse(6), soil%zshh(7), bgc%ratecp(3),
+   bgc%ratecs(2) (src/offline/cable_define_types.F90:119-213, 706-717). */
+int cable_gpu_set_globals(cable_gpu_ctx *ctx, const double *zse, const double *zshh,
+                          const double *ratecp, const double *ratecs);
+
+/* One timestep with every diagnostic of the derived types written (the
+   drop-in for cbm; ktau counts from 1, dels in seconds).  The first call of
+   a context performs the reference's first-call initialisation of
+   gammzz(:,1) (src/science/soilsnow/cbl_soilsnow_main.F90:60-62, 92-96). */
+int cable_gpu_step(cable_gpu_ctx *ctx, int ktau, double dels);
+
+/* ---- device-resident run (production / benchmark path) ---------------- */
+/* Per-class parameter tables: veg (17 x cable_gpu_veg_cols()) and soil
+   (9 x cable_gpu_soil_cols()), row-major FP64, in the column order given by
+   cable_gpu_table_column(); the per-tile iveg, isoilm, vlai and albsoil stay
+   per-tile fields. */
+int cable_gpu_veg_cols(void);
+int cable_gpu_soil_cols(void);
+/* column of field `field_id` in the veg (table=0) or soil (table=1) table,
+   -1 if the field is not tabulated */
+int cable_gpu_table_column(int table, int field_id);
+int cable_gpu_set_class_tables(cable_gpu_ctx *ctx, const double *veg, const double *soil);
+
+/* Forcing for nsteps steps kept in device memory: each array is
+   (nsteps, mp), tile fastest.  The remaining met fields follow the offline
+   forcing reader (src/offline/cable_input.F90:1880-1883, 2666-2680):
+   fsd(:,2) = fsd(:,1), tvair = tvrad = tk, precip_sn = precip where
+   tk <= 273.16, ca and doy uniform. */
+int cable_gpu_set_forcing(cable_gpu_ctx *ctx, int nsteps, const void *fsd1, const void *fld,
+                          const void *tk, const void *qv, const void *ua, const void *pmb,
+                          const void *precip, const void *coszen, int host_type, double ca,
+                          double doy);
+
+/* Run steps k0..k0+nsteps-1 of the resident forcing (ktau = k0+1 ...),
+   writing only the prognostic state and the eight headline fluxes
+   (canopy%rnet, fe, fh, fev, fes, fgpp, ssnow%runoff, rad%trad).  Work is
+   queued on the context stream; cable_gpu_sync waits for it. */
+int cable_gpu_run(cable_gpu_ctx *ctx, int k0, int nsteps, double dels);
+int cable_gpu_sync(cable_gpu_ctx *ctx);
+
+/* Device time of the last cable_gpu_run, in milliseconds, measured with HIP
+   events on the context stream around its kernels. */
+int cable_gpu_last_run_ms(cable_gpu_ctx *ctx, double *ms);
+
+/* Total dryLeaf fixed-point passes of the last step or run (summed over tiles
+   and the four stability iterations), a work counter. */
+int cable_gpu_counters(cable_gpu_ctx *ctx, int64_t *dryleaf_passes);
+
+const char *cable_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CABLE_GPU_H */

@@ -1,0 +1,145 @@
+"""GPU parity: the HIP timestep against the FP64 CPU oracle (oracle/), which
+is itself pinned bit-for-bit to the compiled reference (tests/test_oracle.py).
+
+Three levels:
+  * one step from identical state: every field within 1e-10 relative
+    (balance residuals against 1e-10 of the fluxes they close);
+  * the 48-step trajectory: the plain path's drift bounded physically, the
+    mixed path within the 1 W/m2 flux target;
+  * the device-resident run path (class tables in LDS, compact forcing, eight
+    outputs) bit-identical to the full-diagnostic step it shares physics with.
+"""
+import numpy as np
+import pytest
+
+from conftest import snowy
+from cable_amd.fields import FIELDS, BY_NAME
+from cable_amd.workload import Workload
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-10
+
+COMPARE = [f.name for f in FIELDS if f.role in ("STATE", "DIAG") and f.name != "drv%owb"]
+
+# residuals of balance closures: they cancel terms of O(1000) W/m2 (energy) or
+# O(10) mm (water), so their rounding noise is set by those terms, not by the
+# residual itself
+RESID_SCALE = {
+    "bal%ebal": 1e3, "bal%ebal_tot": 1e3, "bal%ebalveg": 1e3, "bal%ebalsoil": 1e3,
+    "bal%drybal": 1e3, "bal%wetbal": 1e3, "bal%radbal": 1e3, "bal%radbalsum": 1e3,
+    "bal%wbal": 1e1, "bal%wbal_tot": 1e1,
+}
+
+HEADLINE = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes",
+            "canopy%fgpp", "ssnow%runoff", "rad%trad"]
+
+
+def _close_enough(g, o, name, rtol):
+    """|g - o| <= rtol*|o| + rtol*scale, scale = the field's largest magnitude
+    over tiles (or the closed terms' magnitude for a residual)."""
+    g = np.asarray(g, dtype=np.float64).reshape(-1)
+    o = np.asarray(o, dtype=np.float64).reshape(-1)
+    scale = RESID_SCALE.get(name, np.abs(o).max() if o.size else 0.0)
+    bound = rtol * (np.abs(o) + scale)
+    bad = np.abs(g - o) > bound
+    if not bad.any():
+        return None
+    i = int(np.argmax(np.abs(g - o) - bound))
+    return f"{name}[{i}] gpu={g[i]!r} oracle={o[i]!r} |d|={abs(g[i]-o[i]):.3e} bound={bound[i]:.3e}"
+
+
+def _upload_forcing(g, forcing, s):
+    from cable_amd import gpu
+    for k, v in forcing.items():
+        if k in gpu.BY_NAME:
+            g.upload(k, v[s])
+
+
+@pytest.mark.parametrize("n,snow", [(153, False), (612, True)])
+def test_each_step_fp64_matches_oracle(n, snow):
+    """Before each checked step the GPU receives the oracle's complete field
+    set, so every step is compared from identical state."""
+    from cable_amd import gpu
+    from oracle.oracle import OracleRun
+    print(f"[parity] fp64 per-step n={n} snow={snow}", flush=True)
+    wl = Workload(n, 48)
+    if snow:
+        snowy(wl)
+    orun = OracleRun(wl)
+    g = gpu.from_workload(wl, precision=gpu.PREC_FP64, full=True)
+    fails = []
+    for s in range(48):
+        fk = {k: v[s] for k, v in wl.forcing.items()}
+        for k, v in fk.items():
+            orun.f[k][...] = v
+        pre = {k: v.copy() for k, v in orun.f.items()}
+        orun.step(fk)
+        g.upload_all(pre)
+        g.cbm(s + 1, wl.dels)
+        for k in COMPARE:
+            msg = _close_enough(g.download(k), orun.f[k], k, TOL_STEP)
+            if msg:
+                fails.append(f"step {s + 1}: {msg}")
+    g.close()
+    assert not fails, f"{len(fails)} field-steps beyond {TOL_STEP}: " + "; ".join(fails[:8])
+
+
+def _trajectories(wl, precision, names, full=True):
+    from cable_amd import gpu
+    g = gpu.from_workload(wl, precision=precision, full=full)
+    out = {k: [] for k in names}
+    for s in range(wl.nsteps):
+        if full:
+            _upload_forcing(g, wl.forcing, s)
+            g.cbm(s + 1, wl.dels)
+        else:
+            g.run(s, 1, wl.dels)
+        for k in names:
+            out[k].append(g.download(k).reshape(-1))
+    passes = g.dryleaf_passes()
+    g.close()
+    return {k: np.stack(v) for k, v in out.items()}, passes
+
+
+# bounds on the 48-step trajectory (absolute, per field)
+TRAJ_FP64 = {"canopy%rnet": 1e-4, "canopy%fe": 1e-4, "canopy%fh": 1e-4, "canopy%fev": 1e-4,
+             "canopy%fes": 1e-4, "canopy%fgpp": 1e-8, "ssnow%runoff": 1e-8, "rad%trad": 1e-5,
+             "ssnow%tgg": 1e-5, "ssnow%wb": 1e-7, "ssnow%snowd": 1e-6, "canopy%cansto": 1e-8}
+# the mixed path's target: fluxes within 1 W/m2, carbon within 0.1 umol/m2/s
+TRAJ_MIXED = {"canopy%rnet": 1.0, "canopy%fe": 1.0, "canopy%fh": 1.0, "canopy%fev": 1.0,
+              "canopy%fes": 1.0, "canopy%fgpp": 0.1, "ssnow%runoff": 1e-3, "rad%trad": 0.1,
+              "ssnow%tgg": 0.05, "ssnow%wb": 1e-3, "ssnow%snowd": 1e-2, "canopy%cansto": 1e-3}
+
+
+@pytest.mark.parametrize("precision,bounds", [(0, TRAJ_FP64), (1, TRAJ_MIXED)], ids=["fp64", "mixed"])
+def test_trajectory_48_steps(precision, bounds):
+    from oracle.oracle import OracleRun
+    print(f"[parity] 48-step trajectory precision={precision}", flush=True)
+    wl = snowy(Workload(612, 48))
+    names = list(bounds)
+    o, _ = OracleRun(wl).run(nsteps=48, outputs=names)
+    g, _ = _trajectories(wl, precision, names)
+    errs = {k: float(np.abs(g[k] - o[k]).max()) for k in names}
+    print("max |gpu - oracle| over 48 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= bounds[k]}
+    assert not bad, f"beyond bounds: {bad}"
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_resident_run_matches_full_step(precision):
+    """cable_gpu_run (tables in LDS, compact forcing, eight outputs) gives the
+    same bits as cable_gpu_step on the same inputs."""
+    print(f"[parity] resident run vs full step precision={precision}", flush=True)
+    wl = snowy(Workload(612, 12))
+    # the resident path keeps gammzz(:,1) only (the one layer carried between
+    # steps) and derives wbliq from wb - wbice when a step starts
+    state = [f.name for f in FIELDS if f.role == "STATE" and f.name != "ssnow%wbliq"]
+    names = HEADLINE + state
+    a, pa = _trajectories(wl, precision, names, full=True)
+    b, pb = _trajectories(wl, precision, names, full=False)
+    for x in (a, b):
+        x["ssnow%gammzz"] = x["ssnow%gammzz"][:, :wl.n]
+    diff = [k for k in names if not np.array_equal(a[k], b[k])]
+    assert not diff, f"fields differing: {diff[:10]}"
+    assert pa == pb
