@@ -1,0 +1,139 @@
+"""Throughput of the CABLE timestep (cbm) on one MI355X, in tile-steps/s.
+
+A step is one cbm pass over every tile of the synthetic workload: the
+device-resident path (cable_gpu_run) with state, parameters, class tables and
+the run's forcing already in HBM when the timed region starts.  The headline
+line is the plain FP64 path on 262,144 tiles x 48 half-hourly steps; the other
+forms (mixed precision, and FP64 at 1,048,576 tiles, the size the mixed and
+surrogate forms are compared at) are reported beside it under "forms".
+
+  python bench.py [--gpus 1] [--steps 48] [--warmup 4] [--no-forms] [--no-cpu]
+
+Prints one JSON line.  "roofline" rates the kernel against HBM bandwidth
+using the algorithmic bytes per tile-step of the path (state read and
+written once, forcing, per-tile parameters and the eight outputs once);
+"cpu_baseline" times the reference's own cbm (built from its sources) on a
+prefix of the same workload on one host core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "tile-steps/sec on 1 MI355X: hybrid vs plain FP64, at ≤1 W/m² flux error"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per tile-step: prognostic state read and written once,
+# 8 forcing fields and the per-tile parameters read once, 8 outputs written
+ALGO_BYTES = {"fp64": 1016, "mixed": 756}
+CPU_SAMPLE_TILES = 16384
+
+
+def run_form(n, steps, warmup, precision, log):
+    from cable_amd import gpu
+    from cable_amd.workload import Workload
+    nsteps = max(steps, warmup, 1)
+    t = time.perf_counter()
+    wl = Workload(n, nsteps)
+    log(f"[bench] workload {n} tiles x {nsteps} steps generated in {time.perf_counter() - t:.1f}s")
+    prec = gpu.PREC_FP64 if precision == "fp64" else gpu.PREC_MIXED
+    if warmup:
+        g = gpu.from_workload(wl, precision=prec, full=False)
+        g.run(0, warmup, wl.dels)
+        g.sync()
+        g.close()
+    g = gpu.from_workload(wl, precision=prec, full=False)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(0, steps, wl.dels)
+    g.sync()
+    wall = time.perf_counter() - t0
+    ev_ms = g.last_run_ms()
+    passes = g.dryleaf_passes()
+    g.close()
+    per_launch_s = ev_ms * 1e-3 / steps
+    achieved = ALGO_BYTES[precision] * n / per_launch_s / 1e9
+    log(f"[bench] {precision} n={n}: wall {wall * 1e3:.2f} ms for {steps} steps, "
+        f"events {ev_ms:.2f} ms, {n * steps / wall:.4g} tile-steps/s")
+    return {"tiles": n, "steps": steps, "value": n * steps / wall, "ms_per_step": wall * 1e3 / steps,
+            "kernel_ms": ev_ms / steps, "achieved_GBs": achieved, "dryleaf_passes": passes}
+
+
+def cpu_baseline(steps, log):
+    """The reference's cbm (oracle/_ref, compiled from its sources) on the
+    first CPU_SAMPLE_TILES tiles of the workload, one process on one core;
+    the CPU restatement (oracle/) when the reference build is absent."""
+    from cable_amd.workload import Workload
+    wl = Workload(CPU_SAMPLE_TILES, steps)
+    try:
+        from oracle import refrun
+        if refrun.available("r8"):
+            _, sec = refrun.run(wl, steps, [], every_step=False, build="r8")
+            kind, what = "reference", "reference cbm promoted to FP64 (amdflang -O2), driver step loop"
+        else:
+            raise FileNotFoundError
+    except (FileNotFoundError, OSError):
+        from oracle.oracle import OracleRun
+        _, sec = OracleRun(wl).run(nsteps=steps, outputs=[], every_step=False)
+        kind, what = "port", "FP64 C restatement of the reference cbm (gcc -O2)"
+    v = CPU_SAMPLE_TILES * steps / sec
+    log(f"[bench] cpu baseline ({kind}): {sec:.2f}s, {v:.4g} tile-steps/s")
+    return {"value": v, "unit": "tile-steps/s", "cores": 1, "kind": kind,
+            "sample": f"first {CPU_SAMPLE_TILES} tiles of the workload x {steps} steps; {what}; "
+                      f"one process; {sec:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--tiles", type=int, default=262144)
+    ap.add_argument("--no-forms", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.stderr.write("bench.py: this build measures one GPU only (tiles are independent; "
+                         "sharding across GPUs is not part of it). Run with --gpus 1.\n")
+        sys.exit(2)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    head = run_form(args.tiles, args.steps, args.warmup, "fp64", log)
+    forms = {f"fp64_{args.tiles}": head}
+    if not args.no_forms:
+        forms["fp64_1048576"] = run_form(1048576, args.steps, args.warmup, "fp64", log)
+        forms["mixed_1048576"] = run_form(1048576, args.steps, args.warmup, "mixed", log)
+        forms["mixed_vs_fp64_at_1048576"] = forms["mixed_1048576"]["value"] / forms["fp64_1048576"]["value"]
+    cpu = None if args.no_cpu else cpu_baseline(args.steps, log)
+    out = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "tile-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded forcing, parameters from the reference's PFT and soil tables)",
+        "config": {"workload": f"cbm plain FP64, {args.tiles} tiles x {args.steps} half-hourly steps",
+                   "tiles": args.tiles, "precision": "fp64", "path": "device-resident run"},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

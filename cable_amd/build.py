@@ -51,7 +51,9 @@ def build_gpu(verbose=False):
 def build_oracle(with_reference=True, verbose=False):
     targets = ["oracle"]
     if with_reference and os.path.isdir("/root/reference/src"):
-        targets += ["r8", "r4"]
+        # the compiled reference, and the reference driven through the
+        # Fortran drop-in for cbm (needs libcable_gpu.so built first)
+        targets += ["r8", "r4", "gpu8", "gpu4"]
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")] + targets,
                        capture_output=True, text=True)
     if r.returncode != 0:

@@ -281,6 +281,7 @@ StepArgs make_args(const cable_gpu_ctx *c, void **tab, int ktau, double dels, in
   a.ca = c->ca;
   a.doy = c->doy;
   a.npass = c->d_npass;
+  a.driver = 1;
   return a;
 }
 
@@ -425,20 +426,33 @@ int cable_gpu_set_globals(cable_gpu_ctx *c, const double *zse, const double *zsh
   return CABLE_OK;
 }
 
-int cable_gpu_step(cable_gpu_ctx *c, int ktau, double dels) {
-  if (!c) return fail(CABLE_E_ARG, "cable_gpu_step: null context");
-  if (ktau < 1 || !(dels > 0)) return fail(CABLE_E_ARG, "cable_gpu_step: bad ktau %d or dels %g", ktau, dels);
+namespace {
+// one full-diagnostic step: driver = 1 for the offline driver's step, 0 for
+// cbm alone
+int step_full(cable_gpu_ctx *c, int ktau, double dels, int driver, const char *who) {
+  if (!c) return fail(CABLE_E_ARG, "%s: null context", who);
+  if (ktau < 1 || !(dels > 0)) return fail(CABLE_E_ARG, "%s: bad ktau %d or dels %g", who, ktau, dels);
   HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(int), c->stream));
   StepArgs a = make_args(c, c->d_tab, ktau, dels, 0);
+  a.driver = driver;
   hipError_t e = c->precision == CABLE_PREC_MIXED ? launch<float, true, false>(c, a)
                                                    : launch<double, true, false>(c, a);
-  if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_step launch: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return fail(CABLE_E_HIP, "%s launch: %s", who, hipGetErrorString(e));
   int np = 0;
   HIPCHECK(hipMemcpyAsync(&np, c->d_npass, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   c->last_passes = np;
   c->first_call = false;
   return CABLE_OK;
+}
+}  // namespace
+
+int cable_gpu_step(cable_gpu_ctx *c, int ktau, double dels) {
+  return step_full(c, ktau, dels, 1, "cable_gpu_step");
+}
+
+int cable_gpu_cbm(cable_gpu_ctx *c, int ktau, double dels) {
+  return step_full(c, ktau, dels, 0, "cable_gpu_cbm");
 }
 
 int cable_gpu_veg_cols(void) { return VC_N; }

@@ -114,6 +114,8 @@ struct StepArgs {
   int64_t foff;          // element offset of this step in the forcing arrays
   int ktau;
   int first_call;        // first soil_snow call of the run (gammzz(:,1) init)
+  int driver;            // 1: the offline driver's step (its contract around
+                         // cbm and the balance checks); 0: cbm alone
   double dels;
   double zse[MS], zshh[MS + 1], ratecp[3], ratecs[2];
   const double *vtab;    // NVEGCLASS x VC_N   (TABLES)

@@ -1895,9 +1895,12 @@ struct Stepper {
   // :472-618) in FP64
   CBM_HD void run() {
     R dels = (R)a.dels;
-    t.met_ofsd[0] = t.met_fsd[0] + t.met_fsd[1];
-    t.canopy_oldcansto[0] = t.canopy_cansto[0];
-    if (t.veg_iveg[0] >= 14) t.veg_vlai[0] = K(0.);
+    if (a.driver) {
+      // the offline driver before cbm, src/offline/cable_serial.F90:572-575
+      t.met_ofsd[0] = t.met_fsd[0] + t.met_fsd[1];
+      t.canopy_oldcansto[0] = t.canopy_cansto[0];
+      if (t.veg_iveg[0] >= 14) t.veg_vlai[0] = K(0.);
+    }
     if (t.veg_iveg[0] == LAKES && t.ssnow_wb[0] < (double)PV(soil_sfc, 0)) {
       t.ssnow_wbtot1[0] = (R)((t.ssnow_wb[0]) * 1000.0 * a.zse[0]);
       t.ssnow_wb[0] = (double)PV(soil_sfc, 0);
@@ -1929,6 +1932,9 @@ struct Stepper {
     t.canopy_fgpp[0] = -K(1.0) * fpn + t.canopy_frday[0];
     t.canopy_fnee[0] = fpn + t.canopy_frs[0] + t.canopy_frp[0];
     t.canopy_fra[0] = t.canopy_frp[0] + t.canopy_frday[0];
+    if (!a.driver) return;
+    // the offline driver after cbm: unit conversions,
+    // src/offline/cable_serial.F90:602-605, then the balance checks
     t.ssnow_smelt[0] = t.ssnow_smelt[0] * dels;
     t.ssnow_rnof1[0] = t.ssnow_rnof1[0] * dels;
     t.ssnow_rnof2[0] = t.ssnow_rnof2[0] * dels;

@@ -27,7 +27,7 @@ HOST_F64, HOST_F32, HOST_I32 = 0, 1, 2
 EXPORTS = [
     "cable_gpu_nfields", "cable_gpu_field_info", "cable_gpu_field_id",
     "cable_gpu_create", "cable_gpu_destroy", "cable_gpu_upload",
-    "cable_gpu_download", "cable_gpu_set_globals", "cable_gpu_step",
+    "cable_gpu_download", "cable_gpu_set_globals", "cable_gpu_step", "cable_gpu_cbm",
     "cable_gpu_veg_cols", "cable_gpu_soil_cols", "cable_gpu_table_column",
     "cable_gpu_set_class_tables", "cable_gpu_set_forcing", "cable_gpu_run",
     "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters",
@@ -61,6 +61,7 @@ def lib():
         "cable_gpu_download": ([vp, c_int, vp, c_int], c_int),
         "cable_gpu_set_globals": ([vp, vp, vp, vp, vp], c_int),
         "cable_gpu_step": ([vp, c_int, c_double], c_int),
+        "cable_gpu_cbm": ([vp, c_int, c_double], c_int),
         "cable_gpu_veg_cols": ([], c_int),
         "cable_gpu_soil_cols": ([], c_int),
         "cable_gpu_table_column": ([c_int, c_int], c_int),
@@ -169,9 +170,15 @@ class CableGPU:
         self._globals = arrs
         _check(lib().cable_gpu_set_globals(self.h, *[_ptr(a) for a in arrs]))
 
-    # -- one step, every diagnostic (the cbm drop-in) ------------------------
-    def cbm(self, ktau: int, dels: float):
+    # -- one step, every diagnostic ------------------------------------------
+    def step(self, ktau: int, dels: float):
+        """One offline-driver step: cbm with the driver's contract around it
+        and the balance checks (cable_serial.F90:572-605, 725-729)."""
         _check(lib().cable_gpu_step(self.h, int(ktau), float(dels)))
+
+    def cbm(self, ktau: int, dels: float):
+        """cbm alone, as the Fortran drop-in calls it."""
+        _check(lib().cable_gpu_cbm(self.h, int(ktau), float(dels)))
 
     def dryleaf_passes(self):
         v = ctypes.c_int64()

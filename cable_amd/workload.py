@@ -23,6 +23,7 @@ of shape ``(k2, k1, mp)`` (same bytes, tile index fastest).
 """
 from __future__ import annotations
 
+import copy
 import math
 
 import numpy as np
@@ -320,3 +321,38 @@ class Workload:
         f["met%doy"] = np.full((k1 - k0, n), DOY)
         f["met%hod"] = np.tile(hod[:, None], (1, n))
         return f
+
+    # ------------------------------------------------------------------
+    def subset(self, idx):
+        """The same run restricted to tiles ``idx`` (grid-wide parameters such
+        as soil%zse and bgc%ratecp are kept whole)."""
+        idx = np.asarray(idx)
+        n = self.n
+        sub = copy.copy(self)
+        sub.n = int(idx.size)
+
+        def cut(d):
+            return {k: (v[..., idx] if k not in GRID_WIDE and v.shape[-1] == n else v.copy())
+                    for k, v in d.items()}
+        sub.params = cut(self.params)
+        sub.state0 = cut(self.state0)
+        sub.forcing = cut(self.forcing) if self.forcing is not None else None
+        for a in ("iveg", "isoilm", "vlai", "lat", "t0"):
+            setattr(sub, a, getattr(self, a)[idx])
+        sub.albsoil = self.albsoil[:, idx]
+        sub.make_forcing = None
+        return sub
+
+
+# parameters that are one value per grid, not per tile
+GRID_WIDE = ("soil%zse", "soil%zshh", "bgc%ratecp", "bgc%ratecs")
+
+
+def add_snow(wl, every=3, deep=150.0, thin=5.0):
+    """Start a workload's tiles with a mix of deep snow (three layers once
+    snowcheck has run), thin single-layer snow and none, in turn."""
+    sn = np.zeros(wl.n)
+    sn[::every] = deep
+    sn[1::every] = thin
+    wl.state0["ssnow%snowd"] = sn
+    return wl

@@ -80,10 +80,20 @@ int cable_gpu_download(cable_gpu_ctx *ctx, int field_id, void *host, int host_ty
 int cable_gpu_set_globals(cable_gpu_ctx *ctx, const double *zse, const double *zshh,
                           const double *ratecp, const double *ratecs);
 
-/* One timestep with every diagnostic of the derived types written (the
-   drop-in for cbm; ktau counts from 1, dels in seconds).  The first call of
-   a context performs the reference's first-call initialisation of
-   gammzz(:,1) (src/science/soilsnow/cbl_soilsnow_main.F90:60-62, 92-96). */
+/* One call of the reference's cbm (src/offline/cbl_model_driver_offline.F90:
+   38-231) with every diagnostic of the derived types written; ktau counts
+   from 1, dels in seconds.  The first call of a context performs the
+   reference's first-call initialisation of gammzz(:,1)
+   (src/science/soilsnow/cbl_soilsnow_main.F90:60-62, 92-96).  This is what
+   the Fortran drop-in (cable_amd/fortran/cbm_gpu.F90) calls. */
+int cable_gpu_cbm(cable_gpu_ctx *ctx, int ktau, double dels);
+
+/* One step of the offline driver: its contract before cbm (ofsd, oldcansto,
+   vlai = 0 where iveg >= 14; src/offline/cable_serial.F90:572-575), cbm, the
+   unit conversions after it (:602-605) and the per-tile water and energy
+   balance checks (src/offline/cable_checks.F90:472-618, called at
+   cable_serial.F90:725-729; the driver's saved previous water store is the
+   field drv%owb). */
 int cable_gpu_step(cable_gpu_ctx *ctx, int ktau, double dels);
 
 /* ---- device-resident run (production / benchmark path) ---------------- */

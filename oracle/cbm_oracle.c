@@ -2065,3 +2065,9 @@ int64_t oracle_cbm_step(int64_t mp, int ktau, double dels, int first_call,
 }
 
 int oracle_nfields(void) { return CABLE_NFIELDS; }
+
+/* test hook: the Thomas solve of a single kmax-row system (kmax <= 9), in
+   place on rhs */
+void oracle_trimb(const double *a, const double *b, const double *c, double *rhs, int kmax) {
+  trimb(a, b, c, rhs, kmax);
+}

@@ -36,6 +36,8 @@ CHARACTER(LEN=40) :: name
 REAL(8), ALLOCATABLE :: buf(:)
 TYPE (climate_type) :: climate
 REAL, ALLOCATABLE :: c1(:,:), rhoch(:,:), xk(:,:)
+INTEGER(8) :: clk0, clk1, clkrate
+REAL(8) :: step_seconds = 0.0_8
 
 CALL GET_COMMAND_ARGUMENT(1, inpath)
 CALL GET_COMMAND_ARGUMENT(2, outpath)
@@ -101,6 +103,7 @@ DO
     met%ofsd = 0.1
   CASE (3)
     READ(uin) ktau4, dels8
+    CALL SYSTEM_CLOCK(clk0, clkrate)
     dels = REAL(dels8)
     ktau_gl = ktau4
     ! src/offline/cable_serial.F90:572-575
@@ -117,6 +120,8 @@ DO
     ! src/offline/cable_serial.F90:725-729 (check%mass_bal, check%energy_bal)
     CALL drv_mass_balance(dels, ktau4)
     CALL drv_energy_balance()
+    CALL SYSTEM_CLOCK(clk1)
+    step_seconds = step_seconds + REAL(clk1 - clk0, 8) / REAL(clkrate, 8)
   CASE (4)
     READ(uin) name
     n = field_size(name)
@@ -130,6 +135,11 @@ DO
     WRITE(uout) name, n, buf
   CASE (5)
     CALL field_names(6)
+  CASE (6)
+    ! wall time spent in the step operations (cbm and the driver contract
+    ! around it), excluding file input and output
+    name = 'drv%step_seconds'
+    WRITE(uout) name, 1_8, step_seconds
   CASE DEFAULT
     WRITE(0,*) 'bad op ', op
     ERROR STOP 4

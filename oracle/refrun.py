@@ -47,7 +47,9 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
     each step (result[name] has a leading step axis), else after the last.
     poison: optional (step, name, value) -- overwrite a field before that
     step (0-based), used to find which fields carry state between steps.
-    Returns (dict, seconds spent inside the driver process).
+    Returns (dict, seconds the driver spent in its step operations -- cbm
+    and the driver contract around it -- measured inside the driver process,
+    without its file input and output).
     """
     nsteps = wl.nsteps if nsteps is None else nsteps
     with tempfile.TemporaryDirectory() as td:
@@ -71,6 +73,7 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
                 if every_step or s == nsteps - 1:
                     for nm in outputs:
                         f.write(struct.pack("<i", 4) + _name(nm))
+            f.write(struct.pack("<i", 6))
             f.write(struct.pack("<i", 0))
         t0 = time.perf_counter()
         subprocess.run([driver_path(build), pin, pout], check=True,
@@ -86,7 +89,8 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
             arr = np.frombuffer(data, dtype=np.float64, count=n, offset=pos + 48).copy()
             pos += 48 + 8 * n
             res.setdefault(nm, []).append(arr)
+    step_seconds = float(res.pop("drv%step_seconds")[0][0])
     out = {}
     for nm, lst in res.items():
         out[nm] = np.stack(lst) if every_step else lst[-1]
-    return out, dt
+    return out, step_seconds

@@ -1,5 +1,6 @@
 import os
 import sys
+from types import SimpleNamespace
 
 import numpy as np
 import pytest
@@ -13,14 +14,28 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libcable_gpu.so)")
 
 
-def snowy(wl, every=3, deep=150.0, thin=5.0):
-    """Give a workload's tiles a mix of deep (3-layer) snow, thin snow and none."""
-    n = wl.n
-    sn = np.zeros(n)
-    sn[::every] = deep
-    sn[1::every] = thin
-    wl.state0["ssnow%snowd"] = sn
-    return wl
+from cable_amd.workload import add_snow as snowy  # noqa: E402
+
+GOLDEN = os.path.join(ROOT, "tests", "golden", "cbm_ref_77x8.npz")
+
+
+def golden():
+    """The reference's golden vectors (tests/golden/make_golden.py) as
+    (case, expected): ``case`` has the attributes of a Workload that the
+    oracle and the GPU wrapper read; ``expected[prefix][field]`` holds the
+    reference outputs (prefix r8, r8h, r4h, r4)."""
+    z = np.load(GOLDEN)
+    inputs = {k[3:]: z[k] for k in z.files if k.startswith("in:")}
+    forcing = {k[2:]: z[k] for k in z.files if k.startswith("f:")}
+    case = SimpleNamespace(n=int(inputs["veg%iveg"].size), nsteps=int(z["meta:nsteps"]),
+                           dels=float(z["meta:dels"]), params=inputs, state0={},
+                           forcing=forcing)
+    expected = {}
+    for k in z.files:
+        pre, _, name = k.partition(":")
+        if pre in ("r8", "r8h", "r4h", "r4"):
+            expected.setdefault(pre, {})[name] = z[k]
+    return case, expected
 
 
 def rel_err(a, b):

@@ -1,0 +1,74 @@
+"""Write the golden vectors of cbm() from the compiled reference.
+
+Runs the reference's own cbm() (built from the sources under /root/reference
+by oracle/Makefile into oracle/_ref/, once promoted to FP64 and once at its
+shipped default-REAL precision) on a small synthetic case and stores inputs
+and outputs as a fixture:
+
+  tests/golden/cbm_ref_77x8.npz
+    meta:dels, meta:nsteps
+    in:<field>     parameters and initial state (reference layout, float64)
+    f:<field>      forcing, leading axis = step
+    r8:<field>     every STATE/DIAG field after steps 1 and 8 (axis 0 = [1, 8])
+    r8h:<field>    the eight headline fluxes after every step
+    r4h:<field>    the same from the shipped-precision build
+    r4:<field>     prognostic state after step 8, shipped-precision build
+
+The case: tiles 0, 2, 4, ..., 152 of the 153-tile workload (every PFT, every
+soil class, each of deep / thin / no initial snow), 8 half-hourly steps.
+
+Run:  python tests/golden/make_golden.py   (needs oracle/_ref built: make -C oracle r8 r4)
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from cable_amd.fields import FIELDS  # noqa: E402
+from cable_amd.workload import Workload, add_snow  # noqa: E402
+from oracle import refrun  # noqa: E402
+
+OUT = os.path.join(HERE, "cbm_ref_77x8.npz")
+NSTEPS = 8
+HEADLINE = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes",
+            "canopy%fgpp", "ssnow%runoff", "rad%trad"]
+
+
+def case():
+    wl = add_snow(Workload(153, NSTEPS))
+    return wl.subset(np.arange(0, 153, 2))
+
+
+def main():
+    for b in ("r8", "r4"):
+        if not refrun.available(b):
+            sys.exit(f"oracle/_ref/{b} is not built (make -C oracle {b})")
+    wl = case()
+    names = [f.name for f in FIELDS if f.role in ("STATE", "DIAG") and not f.name.startswith("drv%")]
+    r8, _ = refrun.run(wl, NSTEPS, names, every_step=True, build="r8")
+    r4, _ = refrun.run(wl, NSTEPS, names, every_step=True, build="r4")
+    out = {"meta:dels": np.array(wl.dels), "meta:nsteps": np.array(NSTEPS)}
+    for k, v in wl.params.items():
+        out["in:" + k] = v
+    for k, v in wl.state0.items():
+        out["in:" + k] = v
+    for k, v in wl.forcing.items():
+        out["f:" + k] = v
+    for k in names:
+        out["r8:" + k] = r8[k][[0, NSTEPS - 1]]
+    for k in HEADLINE:
+        out["r8h:" + k] = r8[k]
+        out["r4h:" + k] = r4[k]
+    for f in FIELDS:
+        if f.role == "STATE" and f.name in names:
+            out["r4:" + f.name] = r4[f.name][NSTEPS - 1]
+    np.savez_compressed(OUT, **out)
+    print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {wl.n} tiles x {NSTEPS} steps)")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,77 @@
+"""The Fortran drop-in for cbm (cable_amd/fortran/cbm_gpu.F90).
+
+oracle/_ref/gpu8 and gpu4 are the reference's own cbm closure and driver
+built with cbm_gpu.F90 in place of src/offline/cbl_model_driver_offline.F90,
+so there the reference's code calls the HIP library through the
+ISO_C_BINDING layer.  The GPU tests run them on the golden case and compare
+with the reference's own outputs stored in tests/golden.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from cable_amd.fields import FIELDS
+from oracle import refrun
+
+REF_NAMES = [f.name for f in FIELDS if f.role in ("STATE", "DIAG") and not f.name.startswith("drv%")]
+RESID = ("bal%ebal", "bal%ebal_tot", "bal%ebalveg", "bal%ebalsoil", "bal%drybal", "bal%wetbal",
+         "bal%radbal", "bal%radbalsum", "bal%wbal", "bal%wbal_tot")
+
+
+def test_generated_bindings_are_current():
+    from cable_amd.fortran import gen_fields
+    with open(gen_fields.OUT) as fh:
+        assert fh.read() == gen_fields.generate(), "run python cable_amd/fortran/gen_fields.py"
+
+
+def test_bindings_cover_every_field():
+    from cable_amd.fortran import gen_fields
+    txt = gen_fields.generate()
+    for f in FIELDS:
+        if f.name.startswith("drv%"):
+            continue
+        assert f"C_LOC({f.name})" in txt, f.name
+
+
+def _scaled_err(got, exp, name):
+    got = np.asarray(got, dtype=np.float64).reshape(-1)
+    exp = np.asarray(exp, dtype=np.float64).reshape(-1)
+    scale = 1e3 if name in RESID else max(np.abs(exp).max(), 1e-300)
+    return float(np.max(np.abs(got - exp) / (np.abs(exp) + scale))) if exp.size else 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refrun.available("gpu8"), reason="oracle/_ref/gpu8 not built")
+def test_reference_driver_through_fortran_dropin_fp64():
+    """Step 1 of the promoted reference driver with the GPU cbm matches the
+    CPU reference's golden outputs within 1e-10; step 8 within the drift
+    bounds of the plain path."""
+    print("[fortran] gpu8 driver on the golden case", flush=True)
+    case, exp = golden()
+    out, _ = refrun.run(case, case.nsteps, REF_NAMES, every_step=True, build="gpu8", timeout=300)
+    worst = sorted(((_scaled_err(out[k][0], exp["r8"][k][0], k), k) for k in REF_NAMES), reverse=True)
+    print("worst step-1 fields:", worst[:5], flush=True)
+    assert worst[0][0] <= 1e-10, worst[:5]
+    for k, bound in (("canopy%fe", 1e-4), ("canopy%fh", 1e-4), ("canopy%rnet", 1e-4),
+                     ("ssnow%tgg", 1e-5), ("ssnow%wb", 1e-7)):
+        d = np.abs(out[k][case.nsteps - 1] - exp["r8"][k][1]).max()
+        assert d <= bound, (k, d)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refrun.available("gpu4"), reason="oracle/_ref/gpu4 not built")
+def test_shipped_precision_driver_through_fortran_dropin():
+    """The shipped-precision build (host REAL = FP32) with the GPU cbm: the
+    device computes in FP64 from the FP32 fields, so its fluxes sit within
+    the FP32 reference's own distance from the FP64 one (1 W/m2)."""
+    print("[fortran] gpu4 driver on the golden case", flush=True)
+    case, exp = golden()
+    names = list(exp["r8h"])
+    out, _ = refrun.run(case, case.nsteps, names, every_step=True, build="gpu4", timeout=300)
+    for k in ("canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes"):
+        d_gpu = np.abs(out[k] - exp["r8h"][k]).max()
+        d_r4 = np.abs(exp["r4h"][k] - exp["r8h"][k]).max()
+        print(f"{k}: |gpu4 - r8| {d_gpu:.3e}  |r4 - r8| {d_r4:.3e}", flush=True)
+        assert d_gpu <= 1.0, (k, d_gpu)

@@ -76,7 +76,7 @@ def test_each_step_fp64_matches_oracle(n, snow):
         pre = {k: v.copy() for k, v in orun.f.items()}
         orun.step(fk)
         g.upload_all(pre)
-        g.cbm(s + 1, wl.dels)
+        g.step(s + 1, wl.dels)
         for k in COMPARE:
             msg = _close_enough(g.download(k), orun.f[k], k, TOL_STEP)
             if msg:
@@ -92,7 +92,7 @@ def _trajectories(wl, precision, names, full=True):
     for s in range(wl.nsteps):
         if full:
             _upload_forcing(g, wl.forcing, s)
-            g.cbm(s + 1, wl.dels)
+            g.step(s + 1, wl.dels)
         else:
             g.run(s, 1, wl.dels)
         for k in names:
@@ -129,7 +129,7 @@ def test_trajectory_48_steps(precision, bounds):
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 def test_resident_run_matches_full_step(precision):
     """cable_gpu_run (tables in LDS, compact forcing, eight outputs) gives the
-    same bits as cable_gpu_step on the same inputs."""
+    same bits as cable_gpu_step (the offline driver step) on the same inputs."""
     print(f"[parity] resident run vs full step precision={precision}", flush=True)
     wl = snowy(Workload(612, 12))
     # the resident path keeps gammzz(:,1) only (the one layer carried between
