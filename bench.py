@@ -96,7 +96,21 @@ def main():
     ap.add_argument("--tiles", type=int, default=262144)
     ap.add_argument("--no-forms", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES bytes (torch), "
+                         "the reference kernel for scaling HBM counters, and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
+                          "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         sys.stderr.write("bench.py: this build measures one GPU only (tiles are independent; "
                          "sharding across GPUs is not part of it). Run with --gpus 1.\n")
@@ -105,13 +119,14 @@ def main():
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    # the CPU leg first, while this process has not touched the GPU
+    cpu = None if args.no_cpu else cpu_baseline(args.steps, log)
     head = run_form(args.tiles, args.steps, args.warmup, "fp64", log)
     forms = {f"fp64_{args.tiles}": head}
     if not args.no_forms:
         forms["fp64_1048576"] = run_form(1048576, args.steps, args.warmup, "fp64", log)
         forms["mixed_1048576"] = run_form(1048576, args.steps, args.warmup, "mixed", log)
         forms["mixed_vs_fp64_at_1048576"] = forms["mixed_1048576"]["value"] / forms["fp64_1048576"]["value"]
-    cpu = None if args.no_cpu else cpu_baseline(args.steps, log)
     out = {
         "metric": METRIC,
         "value": head["value"],

@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(256) cbm_kernel(StepArgs a) {
   if (a.npass) {
     // wave-level sum, one atomic per wave
     for (int o = 32; o > 0; o >>= 1) passes += __shfl_xor(passes, o, 64);
-    if ((threadIdx.x & 63) == 0 && passes) atomicAdd(a.npass, passes);
+    if ((threadIdx.x & 63) == 0 && passes) atomicAdd(a.npass, (unsigned long long)passes);
   }
 }
 
@@ -250,7 +250,7 @@ struct cable_gpu_ctx {
   double *d_vtab = nullptr, *d_stab = nullptr;
   bool have_tables = false;
   bool first_call = true;
-  int *d_npass = nullptr;
+  unsigned long long *d_npass = nullptr;
   int64_t last_passes = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -361,7 +361,7 @@ int cable_gpu_create(cable_gpu_ctx **out, int64_t mp, int precision) {
   if (e == hipSuccess) e = hipMalloc(&c->d_tab_run, sizeof(void *) * CABLE_NFIELDS);
   if (e == hipSuccess)
     e = hipMemcpyAsync(c->d_tab, c->dev.data(), sizeof(void *) * CABLE_NFIELDS, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = hipMalloc(&c->d_npass, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&c->d_npass, sizeof(unsigned long long));
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
     cable_gpu_destroy(c);
@@ -432,16 +432,16 @@ namespace {
 int step_full(cable_gpu_ctx *c, int ktau, double dels, int driver, const char *who) {
   if (!c) return fail(CABLE_E_ARG, "%s: null context", who);
   if (ktau < 1 || !(dels > 0)) return fail(CABLE_E_ARG, "%s: bad ktau %d or dels %g", who, ktau, dels);
-  HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(int), c->stream));
+  HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
   StepArgs a = make_args(c, c->d_tab, ktau, dels, 0);
   a.driver = driver;
   hipError_t e = c->precision == CABLE_PREC_MIXED ? launch<float, true, false>(c, a)
                                                    : launch<double, true, false>(c, a);
   if (e != hipSuccess) return fail(CABLE_E_HIP, "%s launch: %s", who, hipGetErrorString(e));
-  int np = 0;
-  HIPCHECK(hipMemcpyAsync(&np, c->d_npass, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  unsigned long long np = 0;
+  HIPCHECK(hipMemcpyAsync(&np, c->d_npass, sizeof np, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  c->last_passes = np;
+  c->last_passes = (int64_t)np;
   c->first_call = false;
   return CABLE_OK;
 }
@@ -516,10 +516,10 @@ int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
   if (k0 < 0 || nsteps <= 0 || k0 + nsteps > c->nsteps_forc || !(dels > 0))
     return fail(CABLE_E_ARG, "cable_gpu_run: steps %d..%d outside the %d resident forcing steps", k0,
                 k0 + nsteps - 1, c->nsteps_forc);
+  HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   for (int k = k0; k < k0 + nsteps; k++) {
     StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
-    a.npass = nullptr;
     hipError_t e = c->precision == CABLE_PREC_MIXED ? launch<float, false, true>(c, a)
                                                      : launch<double, false, true>(c, a);
     if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run launch: %s", hipGetErrorString(e));
@@ -530,6 +530,9 @@ int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
   float ms = 0.f;
   HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  unsigned long long np = 0;
+  HIPCHECK(hipMemcpy(&np, c->d_npass, sizeof np, hipMemcpyDeviceToHost));
+  c->last_passes = (int64_t)np;
   return CABLE_OK;
 }
 

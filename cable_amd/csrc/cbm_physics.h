@@ -123,7 +123,7 @@ struct StepArgs {
   int compact_forcing;   // FAST mode: fsd2 = fsd1, tvrad = tvair = tk,
                          // precip_sn from tk, ca and doy scalars
   double ca, doy;
-  int *npass;            // optional: dryLeaf passes per tile (diagnostic)
+  unsigned long long *npass;  // optional: sum of dryLeaf passes (work counter)
 };
 
 // literal of the working precision

@@ -882,8 +882,8 @@ struct Stepper {
     t.canopy_fns_cor[0] = K(0.0);
     t.canopy_ga_cor[0] = K(0.0);
     radiation(sunlit_veg);
-    t.canopy_zetar[0] = ZETA0;
-    t.canopy_zetar[1] = ZETPOS + K(1);
+    // stability parameter of each NITER pass, kept in registers (zetar(:,1..4))
+    R zeta1 = ZETPOS + K(1), zeta2 = t.canopy_zetar[2], zeta3 = t.canopy_zetar[3];
     t.canopy_zetash[0] = ZETA0;
     t.canopy_zetash[1] = ZETPOS + K(1);
     c.sum_rad_rniso = t.rad_rniso[0] + t.rad_rniso[1];
@@ -895,10 +895,7 @@ struct Stepper {
     const R tvrad = t.met_tvrad[0], transd = t.rad_transd[0];
     for (int iter = 1; iter <= 4; iter++) {
       // comp_friction_vel, src/science/canopy/cbl_friction_vel.F90:19-108
-      R zr = t.canopy_zetar[0];
-      if (iter == 2) zr = t.canopy_zetar[1];
-      if (iter == 3) zr = t.canopy_zetar[2];
-      if (iter == 4) zr = t.canopy_zetar[3];
+      const R zr = iter == 1 ? ZETA0 : iter == 2 ? zeta1 : iter == 3 ? zeta2 : zeta3;
       R psim_1 = psim(zr * zuv / ztq);
       R rescale = VONK * MAX(t.met_ua[0], UMIN);
       R z_eff = zuv / z0m;
@@ -995,11 +992,15 @@ struct Stepper {
               (t.air_rho[0] * CAPP * t.met_tk[0] * p3(us));
         z = MIN(ZETPOS, z);
         z = MAX(ZETNEG, z);
-        if (iterplus == 2) t.canopy_zetar[1] = z;
-        if (iterplus == 3) t.canopy_zetar[2] = z;
-        if (iterplus == 4) t.canopy_zetar[3] = z;
+        if (iterplus == 2) zeta1 = z;
+        if (iterplus == 3) zeta2 = z;
+        if (iterplus == 4) zeta3 = z;
       }
     }
+    t.canopy_zetar[0] = ZETA0;
+    t.canopy_zetar[1] = zeta1;
+    t.canopy_zetar[2] = zeta2;
+    t.canopy_zetar[3] = zeta3;
     R us = t.canopy_us[0], ua = t.met_ua[0];
     R rho = t.air_rho[0];
     R rlam = t.air_rlam[0];

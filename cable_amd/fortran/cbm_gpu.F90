@@ -14,16 +14,21 @@
 ! into the derived types.  The caller's c1, rhoch and xk scratch arrays are
 ! not used: the kernel keeps those coefficients in registers.
 !
-! Device precision: FP64 by default; setting cable_gpu_precision to
-! CABLE_PREC_MIXED before the first call selects the mixed path (FP32
-! radiation, roughness and canopy iteration; FP64 soil solves).
+! Device precision follows the host build unless cable_gpu_precision is set
+! before the first call: a build with 8-byte default REAL (-fdefault-real-8)
+! gets the FP64 path; the shipped build (default REAL FP32, REAL(r_2) FP64)
+! gets the mixed path, which keeps exactly that split on the device.  FP64
+! arithmetic on FP32-stored fields is not offered: the reference compares
+! some carried values exactly (e.g. ssnow%cls against 1.1335 in
+! src/science/soilsnow/cbl_snowAccum.F90), and those tests flip when an FP64
+! value is rounded to FP32 between steps.
 MODULE cable_cbm_module
 USE, INTRINSIC :: ISO_C_BINDING
 USE cable_gpu_c
 IMPLICIT NONE
 PRIVATE
 PUBLIC :: cbm, cbm_gpu_finalise
-INTEGER(C_INT), PUBLIC, SAVE :: cable_gpu_precision = CABLE_PREC_FP64
+INTEGER(C_INT), PUBLIC, SAVE :: cable_gpu_precision = -1   ! -1: from KIND(1.0)
 
 TYPE(C_PTR), SAVE :: ctx = C_NULL_PTR
 
@@ -56,6 +61,8 @@ SUBROUTINE cbm(ktau, dels, air, bgc, canopy, met, bal, rad, rough, soil, &
   REAL(C_DOUBLE) :: zse(ms), zshh(ms+1), ratecp(ncp), ratecs(ncs)
 
   IF (.NOT. C_ASSOCIATED(ctx)) THEN
+    IF (cable_gpu_precision < 0) &
+      cable_gpu_precision = MERGE(CABLE_PREC_FP64, CABLE_PREC_MIXED, KIND(1.0) == 8)
     CALL cable_gpu_check(cable_gpu_create(ctx, INT(mp, C_INT64_T), cable_gpu_precision))
     zse = soil%zse
     zshh = soil%zshh

@@ -120,8 +120,9 @@ int cable_gpu_set_forcing(cable_gpu_ctx *ctx, int nsteps, const void *fsd1, cons
 
 /* Run steps k0..k0+nsteps-1 of the resident forcing (ktau = k0+1 ...),
    writing only the prognostic state and the eight headline fluxes
-   (canopy%rnet, fe, fh, fev, fes, fgpp, ssnow%runoff, rad%trad).  Work is
-   queued on the context stream; cable_gpu_sync waits for it. */
+   (canopy%rnet, fe, fh, fev, fes, fgpp, ssnow%runoff, rad%trad).  The
+   launches are queued on the context stream and the call returns when they
+   have completed (it reads the device time and the work counter). */
 int cable_gpu_run(cable_gpu_ctx *ctx, int k0, int nsteps, double dels);
 int cable_gpu_sync(cable_gpu_ctx *ctx);
 

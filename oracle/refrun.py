@@ -8,6 +8,7 @@ src/science/soilsnow/cbl_soilsnow_main.F90:60-62, so one process is one run).
 from __future__ import annotations
 
 import os
+import resource
 import struct
 import subprocess
 import tempfile
@@ -17,6 +18,11 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
+
+
+def _big_stack():
+    _, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
 
 
 def driver_path(build: str = "r8") -> str:
@@ -76,8 +82,11 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
             f.write(struct.pack("<i", 6))
             f.write(struct.pack("<i", 0))
         t0 = time.perf_counter()
+        # the reference keeps its per-step temporaries on the stack (automatic
+        # arrays of mp elements), so a large mp needs the stack limit raised,
+        # as the offline model is run with `ulimit -s unlimited`
         subprocess.run([driver_path(build), pin, pout], check=True,
-                       timeout=timeout, stdout=subprocess.DEVNULL)
+                       timeout=timeout, stdout=subprocess.DEVNULL, preexec_fn=_big_stack)
         dt = time.perf_counter() - t0
         res = {}
         with open(pout, "rb") as f:

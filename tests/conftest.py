@@ -16,20 +16,22 @@ def pytest_configure(config):
 
 from cable_amd.workload import add_snow as snowy  # noqa: E402
 
-GOLDEN = os.path.join(ROOT, "tests", "golden", "cbm_ref_77x8.npz")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "cbm_ref_77x28.npz")
 
 
 def golden():
     """The reference's golden vectors (tests/golden/make_golden.py) as
     (case, expected): ``case`` has the attributes of a Workload that the
     oracle and the GPU wrapper read; ``expected[prefix][field]`` holds the
-    reference outputs (prefix r8, r8h, r4h, r4)."""
+    reference outputs (prefix r8: all fields after the steps in
+    ``case.full_steps``; r8h, r4h: headline fluxes after every step; r4:
+    state after the last step)."""
     z = np.load(GOLDEN)
     inputs = {k[3:]: z[k] for k in z.files if k.startswith("in:")}
     forcing = {k[2:]: z[k] for k in z.files if k.startswith("f:")}
     case = SimpleNamespace(n=int(inputs["veg%iveg"].size), nsteps=int(z["meta:nsteps"]),
                            dels=float(z["meta:dels"]), params=inputs, state0={},
-                           forcing=forcing)
+                           forcing=forcing, full_steps=[int(x) for x in z["meta:full_steps"]])
     expected = {}
     for k in z.files:
         pre, _, name = k.partition(":")

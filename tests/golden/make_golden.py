@@ -5,17 +5,18 @@ by oracle/Makefile into oracle/_ref/, once promoted to FP64 and once at its
 shipped default-REAL precision) on a small synthetic case and stores inputs
 and outputs as a fixture:
 
-  tests/golden/cbm_ref_77x8.npz
-    meta:dels, meta:nsteps
+  tests/golden/cbm_ref_77x28.npz
+    meta:dels, meta:nsteps, meta:full_steps
     in:<field>     parameters and initial state (reference layout, float64)
     f:<field>      forcing, leading axis = step
-    r8:<field>     every STATE/DIAG field after steps 1 and 8 (axis 0 = [1, 8])
+    r8:<field>     every STATE/DIAG field after steps 1, 14 and 28 (axis 0)
     r8h:<field>    the eight headline fluxes after every step
     r4h:<field>    the same from the shipped-precision build
-    r4:<field>     prognostic state after step 8, shipped-precision build
+    r4:<field>     prognostic state after the last step, shipped-precision build
 
 The case: tiles 0, 2, 4, ..., 152 of the 153-tile workload (every PFT, every
-soil class, each of deep / thin / no initial snow), 8 half-hourly steps.
+soil class, each of deep / thin / no initial snow), 28 half-hourly steps
+from midnight (night, sunrise at step 13, the morning to 13:30).
 
 Run:  python tests/golden/make_golden.py   (needs oracle/_ref built: make -C oracle r8 r4)
 """
@@ -32,8 +33,9 @@ from cable_amd.fields import FIELDS  # noqa: E402
 from cable_amd.workload import Workload, add_snow  # noqa: E402
 from oracle import refrun  # noqa: E402
 
-OUT = os.path.join(HERE, "cbm_ref_77x8.npz")
-NSTEPS = 8
+OUT = os.path.join(HERE, "cbm_ref_77x28.npz")
+NSTEPS = 28
+FULL_STEPS = [1, 14, 28]
 HEADLINE = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes",
             "canopy%fgpp", "ssnow%runoff", "rad%trad"]
 
@@ -51,7 +53,8 @@ def main():
     names = [f.name for f in FIELDS if f.role in ("STATE", "DIAG") and not f.name.startswith("drv%")]
     r8, _ = refrun.run(wl, NSTEPS, names, every_step=True, build="r8")
     r4, _ = refrun.run(wl, NSTEPS, names, every_step=True, build="r4")
-    out = {"meta:dels": np.array(wl.dels), "meta:nsteps": np.array(NSTEPS)}
+    out = {"meta:dels": np.array(wl.dels), "meta:nsteps": np.array(NSTEPS),
+           "meta:full_steps": np.array(FULL_STEPS)}
     for k, v in wl.params.items():
         out["in:" + k] = v
     for k, v in wl.state0.items():
@@ -59,7 +62,7 @@ def main():
     for k, v in wl.forcing.items():
         out["f:" + k] = v
     for k in names:
-        out["r8:" + k] = r8[k][[0, NSTEPS - 1]]
+        out["r8:" + k] = r8[k][[s - 1 for s in FULL_STEPS]]
     for k in HEADLINE:
         out["r8h:" + k] = r8[k]
         out["r4h:" + k] = r4[k]

@@ -1,7 +1,7 @@
 """The CPU oracle (oracle/cbm_oracle.c) pinned against the reference.
 
 * bit-for-bit against the golden vectors written by the compiled reference
-  (tests/golden/cbm_ref_77x8.npz, tests/golden/make_golden.py);
+  (tests/golden/cbm_ref_77x28.npz, tests/golden/make_golden.py);
 * bit-for-bit against the compiled reference itself on a 612-tile, 48-step
   run with deep, thin and no snow, when oracle/_ref is built (here only);
 * the Thomas solve against a dense solve, and the balance closures the
@@ -35,7 +35,7 @@ def test_oracle_reproduces_golden_vectors():
     out, _ = OracleRun(case).run(nsteps=case.nsteps, outputs=REF_NAMES)
     bad = []
     for k in REF_NAMES:
-        got = out[k][[0, case.nsteps - 1]]
+        got = out[k][[i - 1 for i in case.full_steps]]
         if _mismatch(got, exp["r8"][k]):
             bad.append(k)
     for k, v in exp["r8h"].items():
@@ -94,13 +94,14 @@ def test_balances_close_in_fp64():
 
 
 def test_oracle_catches_a_planted_mistake():
-    """The golden comparison is sensitive: a one-ulp change to one input of
-    one tile shows up in the outputs."""
+    """The golden comparison is sensitive: a 1e-12 relative change to one
+    parameter (vcmax) shows up in the outputs of a daylight step."""
     case, exp = golden()
     case.params = {k: v.copy() for k, v in case.params.items()}
-    case.params["veg%vcmax"][5] = np.nextafter(case.params["veg%vcmax"][5], 1.0)
-    out, _ = OracleRun(case).run(nsteps=1, outputs=["canopy%fpn"])
-    assert _mismatch(out["canopy%fpn"][0], exp["r8"]["canopy%fpn"][0]) >= 1
+    case.params["veg%vcmax"] *= 1.0 + 1e-12
+    step = case.full_steps[1]        # a daylight step
+    out, _ = OracleRun(case).run(nsteps=step, outputs=["canopy%fpn"])
+    assert _mismatch(out["canopy%fpn"][step - 1], exp["r8"]["canopy%fpn"][1]) >= 1
 
 
 def test_workload_is_a_prefix_and_consistent():
