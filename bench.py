@@ -34,7 +34,7 @@ ALGO_BYTES = {"fp64": 1016, "mixed": 756}
 CPU_SAMPLE_TILES = 16384
 
 
-def run_form(n, steps, warmup, precision, log):
+def run_form(n, steps, warmup, precision, log, class_order=True):
     from cable_amd import gpu
     from cable_amd.workload import Workload
     nsteps = max(steps, warmup, 1)
@@ -42,12 +42,13 @@ def run_form(n, steps, warmup, precision, log):
     wl = Workload(n, nsteps)
     log(f"[bench] workload {n} tiles x {nsteps} steps generated in {time.perf_counter() - t:.1f}s")
     prec = gpu.PREC_FP64 if precision == "fp64" else gpu.PREC_MIXED
+    order = gpu.class_order(wl.iveg, wl.isoilm) if class_order else None
     if warmup:
-        g = gpu.from_workload(wl, precision=prec, full=False)
+        g = gpu.from_workload(wl, precision=prec, full=False, order=order)
         g.run(0, warmup, wl.dels)
         g.sync()
         g.close()
-    g = gpu.from_workload(wl, precision=prec, full=False)
+    g = gpu.from_workload(wl, precision=prec, full=False, order=order)
     g.sync()
     t0 = time.perf_counter()
     g.run(0, steps, wl.dels)
@@ -96,6 +97,8 @@ def main():
     ap.add_argument("--tiles", type=int, default=262144)
     ap.add_argument("--no-forms", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--caller-order", action="store_true",
+                    help="keep the tiles in the caller's order on the device (no class grouping)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes (torch), "
                          "the reference kernel for scaling HBM counters, and exit")
@@ -121,11 +124,12 @@ def main():
 
     # the CPU leg first, while this process has not touched the GPU
     cpu = None if args.no_cpu else cpu_baseline(args.steps, log)
-    head = run_form(args.tiles, args.steps, args.warmup, "fp64", log)
+    co = not args.caller_order
+    head = run_form(args.tiles, args.steps, args.warmup, "fp64", log, co)
     forms = {f"fp64_{args.tiles}": head}
     if not args.no_forms:
-        forms["fp64_1048576"] = run_form(1048576, args.steps, args.warmup, "fp64", log)
-        forms["mixed_1048576"] = run_form(1048576, args.steps, args.warmup, "mixed", log)
+        forms["fp64_1048576"] = run_form(1048576, args.steps, args.warmup, "fp64", log, co)
+        forms["mixed_1048576"] = run_form(1048576, args.steps, args.warmup, "mixed", log, co)
         forms["mixed_vs_fp64_at_1048576"] = forms["mixed_1048576"]["value"] / forms["fp64_1048576"]["value"]
     out = {
         "metric": METRIC,
@@ -141,7 +145,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded forcing, parameters from the reference's PFT and soil tables)",
         "config": {"workload": f"cbm plain FP64, {args.tiles} tiles x {args.steps} half-hourly steps",
-                   "tiles": args.tiles, "precision": "fp64", "path": "device-resident run"},
+                   "tiles": args.tiles, "precision": "fp64", "path": "device-resident run",
+                   "tile_order": "caller" if args.caller_order else "grouped by vegetation and soil type"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,

@@ -1,10 +1,10 @@
 // cable_gpu.hip -- kernels and C ABI of libcable_gpu.so (include/cable_gpu.h).
 //
-// One launch of cbm_kernel performs one timestep for every tile: one tile per
-// thread, 256 threads per workgroup, the per-class parameter tables staged
-// into LDS at the start of each workgroup.  All device memory is owned by the
-// context; HBM traffic per tile-step is the tile's state, parameters, forcing
-// and outputs, read and written once.
+// The context: device copies of the fields, resident forcing and class
+// tables, the tile order, the stream and its timing events.  One launch of
+// cbm_kernel (cbm_kernel.h) performs one timestep for every tile; HBM traffic
+// per tile-step is the tile's state, parameters, forcing and outputs, read
+// and written once.
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -15,14 +15,13 @@
 #include <vector>
 
 #include "../../include/cable_gpu.h"
-#include "cbm_step.h"
+#include "cbm_kernel.h"
 
 using namespace cbm;
 
 // ---------------------------------------------------------------------------
 // field table
 namespace {
-enum { ROLE_PARAM = 0, ROLE_FORC = 1, ROLE_STATE = 2, ROLE_DIAG = 3 };
 struct FieldMeta {
   const char *name;
   char prec;
@@ -72,169 +71,6 @@ size_t dev_elem(const FieldMeta &f, int precision) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// kernels
-
-template <class T>
-__device__ __forceinline__ T ld(const void *p, int64_t i) {
-  return static_cast<const T *>(p)[i];
-}
-template <class T, class V>
-__device__ __forceinline__ void st(void *p, int64_t i, V v) {
-  static_cast<T *>(p)[i] = static_cast<T>(v);
-}
-
-// device type of a field of precision code P for working precision R
-template <class R, char P> struct DevT { using type = R; };
-template <class R> struct DevT<R, 'D'> { using type = double; };
-template <class R> struct DevT<R, 'B'> { using type = double; };
-template <class R> struct DevT<R, 'I'> { using type = int; };
-
-constexpr int role_id(int r) { return r; }
-
-template <class R, bool FULL, bool TABLES>
-__global__ void __launch_bounds__(256) cbm_kernel(StepArgs a) {
-  __shared__ double lds_v[TABLES ? NVEGCLASS * VC_N : 1];
-  __shared__ double lds_s[TABLES ? NSOILCLASS * SC_N : 1];
-  if constexpr (TABLES) {
-    for (int i = threadIdx.x; i < NVEGCLASS * VC_N; i += blockDim.x) lds_v[i] = a.vtab[i];
-    for (int i = threadIdx.x; i < NSOILCLASS * SC_N; i += blockDim.x) lds_s[i] = a.stab[i];
-    __syncthreads();
-  }
-  const int64_t mp = a.mp;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int passes = 0;
-  if (t < mp) {
-    void *const *F = a.F;
-    Tile<R> T;
-    // ---- gather ----
-    if constexpr (FULL) {
-#define PARAM ROLE_PARAM
-#define FORC ROLE_FORC
-#define STATE ROLE_STATE
-#define DIAG ROLE_DIAG
-#define X(id, name, prec, n, role)                                                    \
-  {                                                                                   \
-    using DT = typename DevT<R, #prec[0]>::type;                                      \
-    const int64_t off = (role == ROLE_FORC) ? a.foff : 0;                             \
-    _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] =                           \
-        (decltype(T.id[0] + 0))ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t + off);     \
-  }
-      CABLE_FIELDS(X)
-#undef X
-#undef PARAM
-#undef FORC
-#undef STATE
-#undef DIAG
-    } else {
-      // prognostic state
-#define LDS_(id, n)                                                                   \
-  {                                                                                   \
-    using DT = typename DevT<R, 'F'>::type;                                           \
-    _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t); \
-  }
-#define LDD_(id, n)                                                                   \
-  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<double>(F[CABLE_F_##id], (int64_t)k * mp + t); }
-      T.ssnow_isflag[0] = ld<int>(F[CABLE_F_ssnow_isflag], t);
-      LDS_(ssnow_pudsto, 1) LDS_(ssnow_rtsoil, 1) LDS_(ssnow_wbtot1, 1) LDS_(ssnow_wbtot2, 1)
-      LDS_(ssnow_wb_lake, 1) LDS_(ssnow_owetfac, 1) LDS_(ssnow_snage, 1) LDS_(ssnow_snowd, 1)
-      LDS_(ssnow_ssdnn, 1) LDS_(ssnow_tss, 1) LDS_(ssnow_smass, 3) LDS_(ssnow_ssdn, 3)
-      LDS_(ssnow_tgg, 6) LDS_(ssnow_tggsn, 3) LDS_(canopy_cansto, 1) LDS_(bgc_cplant, 3)
-      LDS_(bgc_csoil, 2)
-      LDD_(ssnow_wb, 6) LDD_(ssnow_wbice, 6) LDD_(drv_owb, 1)
-      LDD_(bal_ebal_tot, 1) LDD_(bal_wbal_tot, 1) LDD_(bal_precip_tot, 1) LDD_(bal_rnoff_tot, 1)
-      LDD_(bal_evap_tot, 1) LDD_(bal_radbalsum, 1)
-      T.ssnow_gammzz[0] = ld<double>(F[CABLE_F_ssnow_gammzz], t);
-      // soil liquid water as the previous step left it (wb - wbice), before
-      // this step's lake fix-up
-#pragma unroll
-      for (int k = 0; k < MS; k++) T.ssnow_wbliq[k] = T.ssnow_wb[k] - T.ssnow_wbice[k];
-      // per-tile parameters
-      T.veg_iveg[0] = ld<int>(F[CABLE_F_veg_iveg], t);
-      T.soil_isoilm[0] = ld<int>(F[CABLE_F_soil_isoilm], t);
-      LDS_(veg_vlai, 1)
-      T.soil_albsoil[0] = ld<typename DevT<R, 'F'>::type>(F[CABLE_F_soil_albsoil], t);
-      // compact forcing (offline reader's derivations)
-      {
-        using DT = typename DevT<R, 'F'>::type;
-        const int64_t i = t + a.foff;
-        R fsd = ld<DT>(F[CABLE_F_met_fsd], i);
-        T.met_fsd[0] = fsd;
-        T.met_fsd[1] = fsd;
-        R tk = ld<DT>(F[CABLE_F_met_tk], i);
-        T.met_tk[0] = tk;
-        T.met_tvair[0] = tk;
-        T.met_tvrad[0] = tk;
-        T.met_fld[0] = ld<DT>(F[CABLE_F_met_fld], i);
-        T.met_qv[0] = ld<DT>(F[CABLE_F_met_qv], i);
-        T.met_ua[0] = ld<DT>(F[CABLE_F_met_ua], i);
-        T.met_pmb[0] = ld<DT>(F[CABLE_F_met_pmb], i);
-        R pr = ld<DT>(F[CABLE_F_met_precip], i);
-        T.met_precip[0] = pr;
-        T.met_precip_sn[0] = (tk <= R(273.16)) ? pr : R(0.0);
-        T.met_coszen[0] = ld<DT>(F[CABLE_F_met_coszen], i);
-        T.met_ca[0] = (R)a.ca;
-        T.met_doy[0] = (R)a.doy;
-      }
-#undef LDS_
-#undef LDD_
-    }
-    const int iv = T.veg_iveg[0] - 1, is = T.soil_isoilm[0] - 1;
-    const double *vrow = TABLES ? lds_v + (iv < 0 ? 0 : iv) * VC_N : nullptr;
-    const double *srow = TABLES ? lds_s + (is < 0 ? 0 : is) * SC_N : nullptr;
-    Stepper<R, FULL, TABLES> S(T, a, vrow, srow);
-    S.run();
-    passes = S.npass;
-    // ---- scatter ----
-    if constexpr (FULL) {
-#define PARAM ROLE_PARAM
-#define FORC ROLE_FORC
-#define STATE ROLE_STATE
-#define DIAG ROLE_DIAG
-#define X(id, name, prec, n, role)                                                      \
-  if (role == ROLE_STATE || role == ROLE_DIAG || CABLE_F_##id == CABLE_F_met_tvair ||   \
-      CABLE_F_##id == CABLE_F_veg_vlai) {                                               \
-    using DT = typename DevT<R, #prec[0]>::type;                                        \
-    const int64_t off = (role == ROLE_FORC) ? a.foff : 0;                               \
-    _Pragma("unroll") for (int k = 0; k < n; k++)                                       \
-        st<DT>(F[CABLE_F_##id], (int64_t)k * mp + t + off, T.id[k]);                    \
-  }
-      CABLE_FIELDS(X)
-#undef X
-#undef PARAM
-#undef FORC
-#undef STATE
-#undef DIAG
-    } else {
-      using DT = typename DevT<R, 'F'>::type;
-#define STS_(id, n) \
-  { _Pragma("unroll") for (int k = 0; k < n; k++) st<DT>(F[CABLE_F_##id], (int64_t)k * mp + t, T.id[k]); }
-#define STD_(id, n) \
-  { _Pragma("unroll") for (int k = 0; k < n; k++) st<double>(F[CABLE_F_##id], (int64_t)k * mp + t, T.id[k]); }
-      st<int>(F[CABLE_F_ssnow_isflag], t, T.ssnow_isflag[0]);
-      STS_(ssnow_pudsto, 1) STS_(ssnow_rtsoil, 1) STS_(ssnow_wbtot1, 1) STS_(ssnow_wbtot2, 1)
-      STS_(ssnow_wb_lake, 1) STS_(ssnow_owetfac, 1) STS_(ssnow_snage, 1) STS_(ssnow_snowd, 1)
-      STS_(ssnow_ssdnn, 1) STS_(ssnow_tss, 1) STS_(ssnow_smass, 3) STS_(ssnow_ssdn, 3)
-      STS_(ssnow_tgg, 6) STS_(ssnow_tggsn, 3) STS_(canopy_cansto, 1) STS_(bgc_cplant, 3)
-      STS_(bgc_csoil, 2)
-      STD_(ssnow_wb, 6) STD_(ssnow_wbice, 6) STD_(drv_owb, 1)
-      STD_(bal_ebal_tot, 1) STD_(bal_wbal_tot, 1) STD_(bal_precip_tot, 1) STD_(bal_rnoff_tot, 1)
-      STD_(bal_evap_tot, 1) STD_(bal_radbalsum, 1)
-      st<double>(F[CABLE_F_ssnow_gammzz], t, T.ssnow_gammzz[0]);
-      // the eight headline fluxes
-      STS_(canopy_rnet, 1) STS_(canopy_fe, 1) STS_(canopy_fh, 1) STS_(canopy_fev, 1)
-      STD_(canopy_fes, 1) STS_(canopy_fgpp, 1) STS_(ssnow_runoff, 1) STS_(rad_trad, 1)
-#undef STS_
-#undef STD_
-    }
-  }
-  if (a.npass) {
-    // wave-level sum, one atomic per wave
-    for (int o = 32; o > 0; o >>= 1) passes += __shfl_xor(passes, o, 64);
-    if ((threadIdx.x & 63) == 0 && passes) atomicAdd(a.npass, (unsigned long long)passes);
-  }
-}
-
-// ---------------------------------------------------------------------------
 // context
 
 struct cable_gpu_ctx {
@@ -256,6 +92,9 @@ struct cable_gpu_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   int device = 0;
+  // device position j holds the caller's tile order[j] (empty: identity)
+  std::vector<int64_t> order;
+  bool touched = false;             // device fields hold anything but zeros
 };
 
 namespace {
@@ -285,12 +124,9 @@ StepArgs make_args(const cable_gpu_ctx *c, void **tab, int ktau, double dels, in
   return a;
 }
 
-template <class R, bool FULL, bool TABLES>
-hipError_t launch(const cable_gpu_ctx *c, const StepArgs &a) {
-  const int block = 256;
-  const int64_t grid = (c->mp + block - 1) / block;
-  hipLaunchKernelGGL((cbm_kernel<R, FULL, TABLES>), dim3((unsigned)grid), dim3(block), 0, c->stream, a);
-  return hipGetLastError();
+hipError_t launch(const cable_gpu_ctx *c, const StepArgs &a, bool full) {
+  return c->precision == CABLE_PREC_MIXED ? launch_mixed(full, a, c->mp, c->stream)
+                                          : launch_fp64(full, a, c->mp, c->stream);
 }
 
 // host conversion between a host array and the device element type
@@ -298,12 +134,16 @@ int copy_to_device(cable_gpu_ctx *c, void *dst, const FieldMeta &f, const void *
                    size_t count) {
   size_t de = dev_elem(f, c->precision);
   std::vector<unsigned char> buf(de * count);
+  const size_t mp = (size_t)c->mp;
+  const int64_t *ord = c->order.empty() ? nullptr : c->order.data();
   for (size_t i = 0; i < count; i++) {
+    // rows of mp tiles; device position j of a row takes caller tile ord[j]
+    const size_t s = ord ? (i - i % mp) + (size_t)ord[i % mp] : i;
     double v;
     switch (host_type) {
-      case CABLE_HOST_F64: v = ((const double *)host)[i]; break;
-      case CABLE_HOST_F32: v = ((const float *)host)[i]; break;
-      case CABLE_HOST_I32: v = ((const int32_t *)host)[i]; break;
+      case CABLE_HOST_F64: v = ((const double *)host)[s]; break;
+      case CABLE_HOST_F32: v = ((const float *)host)[s]; break;
+      case CABLE_HOST_I32: v = ((const int32_t *)host)[s]; break;
       default: return fail(CABLE_E_ARG, "bad host type %d", host_type);
     }
     if (f.prec == 'I')
@@ -392,6 +232,7 @@ int cable_gpu_destroy(cable_gpu_ctx *c) {
 
 int cable_gpu_upload(cable_gpu_ctx *c, int id, const void *host, int host_type) {
   if (!c || !host || id < 0 || id >= CABLE_NFIELDS) return fail(CABLE_E_ARG, "cable_gpu_upload: bad arguments");
+  c->touched = true;
   return copy_to_device(c, c->dev[id], kFields[id], host, host_type, (size_t)kFields[id].n * c->mp);
 }
 
@@ -402,16 +243,49 @@ int cable_gpu_download(cable_gpu_ctx *c, int id, void *host, int host_type) {
   std::vector<unsigned char> buf(de * count);
   HIPCHECK(hipMemcpyAsync(buf.data(), c->dev[id], buf.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
+  const size_t mp = (size_t)c->mp;
+  const int64_t *ord = c->order.empty() ? nullptr : c->order.data();
   for (size_t i = 0; i < count; i++) {
+    const size_t d = ord ? (i - i % mp) + (size_t)ord[i % mp] : i;
     double v = f.prec == 'I' ? (double)((int32_t *)buf.data())[i]
                : de == 8     ? ((double *)buf.data())[i]
                              : (double)((float *)buf.data())[i];
     switch (host_type) {
-      case CABLE_HOST_F64: ((double *)host)[i] = v; break;
-      case CABLE_HOST_F32: ((float *)host)[i] = (float)v; break;
-      case CABLE_HOST_I32: ((int32_t *)host)[i] = (int32_t)llround(v); break;
+      case CABLE_HOST_F64: ((double *)host)[d] = v; break;
+      case CABLE_HOST_F32: ((float *)host)[d] = (float)v; break;
+      case CABLE_HOST_I32: ((int32_t *)host)[d] = (int32_t)llround(v); break;
       default: return fail(CABLE_E_ARG, "bad host type %d", host_type);
     }
+  }
+  return CABLE_OK;
+}
+
+int cable_gpu_set_tile_order(cable_gpu_ctx *c, const int64_t *order) {
+  if (!c || !order) return fail(CABLE_E_ARG, "cable_gpu_set_tile_order: null argument");
+  if (c->nsteps_forc)
+    return fail(CABLE_E_STATE, "cable_gpu_set_tile_order: call it before cable_gpu_set_forcing");
+  std::vector<char> seen((size_t)c->mp, 0);
+  for (int64_t j = 0; j < c->mp; j++) {
+    if (order[j] < 0 || order[j] >= c->mp || seen[(size_t)order[j]])
+      return fail(CABLE_E_ARG, "cable_gpu_set_tile_order: not a permutation of 0..%lld (entry %lld)",
+                  (long long)(c->mp - 1), (long long)j);
+    seen[(size_t)order[j]] = 1;
+  }
+  const std::vector<int64_t> old_order = c->order, new_order(order, order + c->mp);
+  if (!c->touched) {  // still zero-filled: nothing to carry over
+    c->order = new_order;
+    return CABLE_OK;
+  }
+  // carry the device contents over to the new order (exact: every device
+  // element type converts to double and back without change)
+  std::vector<double> h;
+  for (int id = 0; id < CABLE_NFIELDS; id++) {
+    h.resize((size_t)kFields[id].n * (size_t)c->mp);
+    c->order = old_order;
+    int r = cable_gpu_download(c, id, h.data(), CABLE_HOST_F64);
+    c->order = new_order;
+    if (!r) r = cable_gpu_upload(c, id, h.data(), CABLE_HOST_F64);
+    if (r) return r;
   }
   return CABLE_OK;
 }
@@ -435,8 +309,8 @@ int step_full(cable_gpu_ctx *c, int ktau, double dels, int driver, const char *w
   HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
   StepArgs a = make_args(c, c->d_tab, ktau, dels, 0);
   a.driver = driver;
-  hipError_t e = c->precision == CABLE_PREC_MIXED ? launch<float, true, false>(c, a)
-                                                   : launch<double, true, false>(c, a);
+  c->touched = true;
+  hipError_t e = launch(c, a, true);
   if (e != hipSuccess) return fail(CABLE_E_HIP, "%s launch: %s", who, hipGetErrorString(e));
   unsigned long long np = 0;
   HIPCHECK(hipMemcpyAsync(&np, c->d_npass, sizeof np, hipMemcpyDeviceToHost, c->stream));
@@ -520,8 +394,7 @@ int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   for (int k = k0; k < k0 + nsteps; k++) {
     StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
-    hipError_t e = c->precision == CABLE_PREC_MIXED ? launch<float, false, true>(c, a)
-                                                     : launch<double, false, true>(c, a);
+    hipError_t e = launch(c, a, false);
     if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run launch: %s", hipGetErrorString(e));
     c->first_call = false;
   }

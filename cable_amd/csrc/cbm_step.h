@@ -1940,8 +1940,10 @@ struct Stepper {
     t.ssnow_rnof1[0] = t.ssnow_rnof1[0] * dels;
     t.ssnow_rnof2[0] = t.ssnow_rnof2[0] * dels;
     t.ssnow_runoff[0] = t.ssnow_runoff[0] * dels;
-    // mass_balance, src/offline/cable_checks.F90:472-551 (FP64 accumulators)
+    // mass_balance, src/offline/cable_checks.F90:472-551 (FP64 accumulators,
+    // evaluated exactly as written in every build of this file)
     {
+#pragma float_control(precise, on)
       if (a.ktau == 1) t.drv_owb[0] = t.ssnow_wbtot[0];
       double delwb = t.ssnow_wbtot[0] - t.drv_owb[0];
       t.drv_owb[0] = t.ssnow_wbtot[0];
@@ -1966,6 +1968,7 @@ struct Stepper {
     }
     // energy_balance, src/offline/cable_checks.F90:565-618
     {
+#pragma float_control(precise, on)
       R *fsd = t.met_fsd, *alb = t.rad_albedo, *qc = t.rad_qcan;
       R tv4 = p4(t.canopy_tv[0]);
       R radbal = fsd[0] + fsd[1] + t.met_fld[0] - alb[0] * fsd[0] - alb[1] * fsd[1] -

@@ -30,7 +30,7 @@ EXPORTS = [
     "cable_gpu_download", "cable_gpu_set_globals", "cable_gpu_step", "cable_gpu_cbm",
     "cable_gpu_veg_cols", "cable_gpu_soil_cols", "cable_gpu_table_column",
     "cable_gpu_set_class_tables", "cable_gpu_set_forcing", "cable_gpu_run",
-    "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters",
+    "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters", "cable_gpu_set_tile_order",
     "cable_gpu_last_error",
 ]
 
@@ -72,6 +72,7 @@ def lib():
         "cable_gpu_last_run_ms": ([vp, ctypes.POINTER(c_double)], c_int),
         "cable_gpu_counters": ([vp, ctypes.POINTER(i64)], c_int),
         "cable_gpu_last_error": ([], ctypes.c_char_p),
+        "cable_gpu_set_tile_order": ([vp, vp], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -165,6 +166,13 @@ class CableGPU:
             if k in BY_NAME:
                 self.upload(k, v)
 
+    def set_tile_order(self, order):
+        """Device position j holds tile order[j] (see class_order)."""
+        o = np.ascontiguousarray(order, dtype=np.int64)
+        if o.shape != (self.mp,):
+            raise CableError(f"tile order of {o.shape} for {self.mp} tiles")
+        _check(lib().cable_gpu_set_tile_order(self.h, _ptr(o)))
+
     def set_globals(self, zse, zshh, ratecp, ratecs):
         arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (zse, zshh, ratecp, ratecs)]
         self._globals = arrs
@@ -215,10 +223,22 @@ class CableGPU:
         _check(lib().cable_gpu_sync(self.h))
 
 
-def from_workload(wl, precision=PREC_FP64, full=True):
+def class_order(iveg, isoilm=None):
+    """A tile order that groups tiles by vegetation type (then soil type):
+    the wavefronts then run one type's canopy iteration counts and branches."""
+    iveg = np.asarray(iveg)
+    if isoilm is None:
+        return np.argsort(iveg, kind="stable")
+    return np.lexsort((np.asarray(isoilm), iveg))
+
+
+def from_workload(wl, precision=PREC_FP64, full=True, order=None):
     """A CableGPU holding workload ``wl``'s parameters and initial state (every
-    other field zero, as the reference's derived types start)."""
+    other field zero, as the reference's derived types start); ``order``: the
+    device tile order (class_order), or None for the caller's order."""
     g = CableGPU(wl.n, precision)
+    if order is not None:
+        g.set_tile_order(order)
     g.upload_all(wl.params)
     g.upload_all(wl.state0)
     p = wl.params

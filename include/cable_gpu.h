@@ -75,6 +75,16 @@ int cable_gpu_destroy(cable_gpu_ctx *ctx);
 int cable_gpu_upload(cable_gpu_ctx *ctx, int field_id, const void *host, int host_type);
 int cable_gpu_download(cable_gpu_ctx *ctx, int field_id, void *host, int host_type);
 
+/* Order of the tiles in device memory: device position j holds the caller's
+   tile order[j] (a permutation of 0..mp-1).  Uploads, downloads and the
+   resident forcing are permuted on the way in and out, so the caller keeps
+   its own tile order; the physics of each tile is unchanged.  Grouping tiles
+   of one vegetation type lets a wavefront run one type's iteration counts
+   and branches instead of all of them.  May be called any time before
+   cable_gpu_set_forcing; device contents are carried over.  (No counterpart
+   in the reference, whose tiles are independent in any order.) */
+int cable_gpu_set_tile_order(cable_gpu_ctx *ctx, const int64_t *order);
+
 /* Grid-wide parameters: soil%zse(6), soil%zshh(7), bgc%ratecp(3),
    bgc%ratecs(2) (src/offline/cable_define_types.F90:119-213, 706-717). */
 int cable_gpu_set_globals(cable_gpu_ctx *ctx, const double *zse, const double *zshh,

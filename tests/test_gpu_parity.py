@@ -85,9 +85,9 @@ def test_each_step_fp64_matches_oracle(n, snow):
     assert not fails, f"{len(fails)} field-steps beyond {TOL_STEP}: " + "; ".join(fails[:8])
 
 
-def _trajectories(wl, precision, names, full=True):
+def _trajectories(wl, precision, names, full=True, order=None):
     from cable_amd import gpu
-    g = gpu.from_workload(wl, precision=precision, full=full)
+    g = gpu.from_workload(wl, precision=precision, full=full, order=order)
     out = {k: [] for k in names}
     for s in range(wl.nsteps):
         if full:
@@ -143,3 +143,22 @@ def test_resident_run_matches_full_step(precision):
     diff = [k for k in names if not np.array_equal(a[k], b[k])]
     assert not diff, f"fields differing: {diff[:10]}"
     assert pa == pb
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_tile_order_is_invisible(precision):
+    """Grouping tiles by class in device memory (cable_gpu_set_tile_order)
+    changes nothing the caller sees: same bits, caller's order."""
+    from cable_amd import gpu
+    print(f"[parity] class tile order precision={precision}", flush=True)
+    wl = snowy(Workload(612, 8))
+    names = HEADLINE + ["ssnow%tgg", "ssnow%wb", "ssnow%snowd", "canopy%cansto"]
+    a, pa = _trajectories(wl, precision, names, full=False)
+    b, pb = _trajectories(wl, precision, names, full=False, order=gpu.class_order(wl.iveg, wl.isoilm))
+    diff = [k for k in names if not np.array_equal(a[k], b[k])]
+    assert not diff, f"fields differing: {diff}"
+    assert pa == pb
+    # the full-diagnostic step and uploads after the reorder
+    c, _ = _trajectories(wl, precision, names, full=True, order=gpu.class_order(wl.iveg))
+    d, _ = _trajectories(wl, precision, names, full=True)
+    assert all(np.array_equal(c[k], d[k]) for k in names)
