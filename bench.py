@@ -65,6 +65,20 @@ def run_form(n, steps, warmup, precision, log, class_order=True):
             "kernel_ms": ev_ms / steps, "achieved_GBs": achieved, "dryleaf_passes": passes}
 
 
+def profiled_traffic(tiles, precision):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    scaled by a plain copy) for this workload, from the committed profile
+    summary of the same kernel, or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"cbm_kernel_{precision}_{tiles}x48.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(steps, log):
     """The reference's cbm (oracle/_ref, compiled from its sources) on the
     first CPU_SAMPLE_TILES tiles of the workload, one process on one core;
@@ -131,6 +145,7 @@ def main():
         forms["fp64_1048576"] = run_form(1048576, args.steps, args.warmup, "fp64", log, co)
         forms["mixed_1048576"] = run_form(1048576, args.steps, args.warmup, "mixed", log, co)
         forms["mixed_vs_fp64_at_1048576"] = forms["mixed_1048576"]["value"] / forms["fp64_1048576"]["value"]
+    traffic, traffic_src = profiled_traffic(args.tiles, "fp64") if not args.caller_order else (None, None)
     out = {
         "metric": METRIC,
         "value": head["value"],
@@ -148,7 +163,8 @@ def main():
                    "tiles": args.tiles, "precision": "fp64", "path": "device-resident run",
                    "tile_order": "caller" if args.caller_order else "grouped by vegetation and soil type"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "forms": forms,
     }
