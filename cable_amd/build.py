@@ -31,6 +31,7 @@ TU_FLAGS = {
     "cable_gpu.hip": ["-ffp-contract=off"],
     "cbm_kernels_fp64.hip": ["-ffp-contract=off"],
     "cbm_kernels_mixed.hip": ["-ffp-contract=on", "-fapprox-func"],
+    "mlp.hip": ["-ffp-contract=on"],
 }
 
 

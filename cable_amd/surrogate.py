@@ -1,0 +1,168 @@
+"""Host side of the canopy surrogate (include/cable_mlp.h): parameter
+initialisation, normalisation statistics and the training loop that drives
+the GPU kernels of cable_amd/csrc/mlp.hip.  Every forward, backward and Adam
+step runs on the GPU; the host only draws the sample order and reads losses.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .gpu import CableError, lib as _gpu_lib
+
+NIN, HID, NOUT = 24, 128, 8
+SHAPES = [(NIN, HID), (HID, HID), (HID, HID), (HID, NOUT)]
+NPARAM = sum(i * o + o for i, o in SHAPES)
+
+_sig_done = False
+
+
+def lib():
+    global _sig_done
+    L = _gpu_lib()
+    if not _sig_done:
+        vp, i64, c_int, c_float = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float
+        sig = {
+            "cable_mlp_nparam": ([], c_int),
+            "cable_mlp_last_error": ([], ctypes.c_char_p),
+            "cable_mlp_create": ([ctypes.POINTER(vp), c_int], c_int),
+            "cable_mlp_destroy": ([vp], c_int),
+            "cable_mlp_set_params": ([vp, vp], c_int),
+            "cable_mlp_get_params": ([vp, vp], c_int),
+            "cable_mlp_set_norm": ([vp, vp], c_int),
+            "cable_mlp_forward": ([vp, vp, c_int, vp], c_int),
+            "cable_mlp_gradient": ([vp, vp, vp, c_int, vp], c_int),
+            "cable_mlp_get_grads": ([vp, vp], c_int),
+            "cable_mlp_load_samples": ([vp, vp, vp, i64], c_int),
+            "cable_mlp_use_samples": ([vp, vp, vp, i64], c_int),
+            "cable_mlp_train": ([vp, vp, i64, c_float, vp], c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        if L.cable_mlp_nparam() != NPARAM:
+            raise CableError("libcable_gpu.so has a different surrogate layout")
+        _sig_done = True
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise CableError(lib().cable_mlp_last_error().decode())
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        raise CableError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def init_params(seed=0):
+    """He-uniform weights (SiLU layers), zero biases, a half-scale output layer."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l, (i, o) in enumerate(SHAPES):
+        lim = np.sqrt(6.0 / i) * (0.5 if l == 3 else 1.0)
+        parts += [rng.uniform(-lim, lim, i * o), np.zeros(o)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def norm_stats(x, y):
+    """mu_x, sd_x, mu_y, sd_y of raw samples (sd floored so constant
+    columns stay finite), packed as the library expects."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    mx, sx = x.mean(0), x.std(0)
+    my, sy = y.mean(0), y.std(0)
+    sx = np.where(sx > 1e-6 * np.maximum(1.0, np.abs(mx)), sx, 1.0)
+    sy = np.where(sy > 1e-6 * np.maximum(1.0, np.abs(my)), sy, 1.0)
+    return np.concatenate([mx, sx, my, sy]).astype(np.float32)
+
+
+class MLP:
+    """One network on the GPU with buffers for batches of ``batch`` rows."""
+
+    def __init__(self, batch=2048, params=None, norm=None):
+        self.batch = int(batch)
+        h = ctypes.c_void_p()
+        _check(lib().cable_mlp_create(ctypes.byref(h), self.batch))
+        self.h = h
+        self.set_params(init_params() if params is None else params)
+        if norm is not None:
+            self.set_norm(norm)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cable_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = _f32(p, (NPARAM,))
+        _check(lib().cable_mlp_set_params(self.h, p.ctypes.data))
+
+    def params(self):
+        out = np.empty(NPARAM, dtype=np.float32)
+        _check(lib().cable_mlp_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_norm(self, norm):
+        self.norm = _f32(norm, (2 * NIN + 2 * NOUT,))
+        _check(lib().cable_mlp_set_norm(self.h, self.norm.ctypes.data))
+
+    def forward(self, x):
+        """Network output for normalised inputs x (n x 24), n <= batch."""
+        x = _f32(x)
+        y = np.empty((x.shape[0], NOUT), dtype=np.float32)
+        _check(lib().cable_mlp_forward(self.h, x.ctypes.data, x.shape[0], y.ctypes.data))
+        return y
+
+    def gradient(self, x, t):
+        x, t = _f32(x), _f32(t)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(lib().cable_mlp_gradient(self.h, x.ctypes.data, t.ctypes.data, x.shape[0], loss.ctypes.data))
+        g = np.empty(NPARAM, dtype=np.float32)
+        _check(lib().cable_mlp_get_grads(self.h, g.ctypes.data))
+        return float(loss[0]), g
+
+    def load_samples(self, x, y):
+        self._xs, self._ys = _f32(x), _f32(y)
+        if self._xs.shape[1] != NIN or self._ys.shape[1] != NOUT or self._xs.shape[0] != self._ys.shape[0]:
+            raise CableError("samples must be (n, 24) and (n, 8)")
+        _check(lib().cable_mlp_load_samples(self.h, self._xs.ctypes.data, self._ys.ctypes.data,
+                                            self._xs.shape[0]))
+        self.nsamples = self._xs.shape[0]
+
+    def use_device_samples(self, x_ptr, y_ptr, n):
+        _check(lib().cable_mlp_use_samples(self.h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), int(n)))
+        self.nsamples = int(n)
+
+    def train(self, epochs=1.0, lr=1e-3, seed=0, lr_final=None):
+        """Adam over shuffled batches of the loaded samples for ``epochs``
+        passes; the learning rate decays linearly to lr_final (default lr/10)
+        in chunks.  Returns the loss of every batch."""
+        rng = np.random.default_rng(seed)
+        n, B = self.nsamples, self.batch
+        nb = max(1, int(round(epochs * n / B)))
+        lr_final = lr / 10.0 if lr_final is None else lr_final
+        losses = np.zeros(nb, dtype=np.float32)
+        done, chunk = 0, 512
+        order = np.empty(0, dtype=np.int64)
+        while done < nb:
+            k = min(chunk, nb - done)
+            while order.size < k * B:
+                order = np.concatenate([order, rng.permutation(n).astype(np.int64)])
+            ob, order = np.ascontiguousarray(order[:k * B]), order[k * B:]
+            rate = lr + (lr_final - lr) * (done + 0.5 * k) / nb
+            _check(lib().cable_mlp_train(self.h, ob.ctypes.data, k, float(rate),
+                                         losses[done:].ctypes.data))
+            done += k
+        return losses

@@ -23,10 +23,10 @@ def L():
     return gpu.lib()
 
 
-def declared():
-    txt = open(HEADER).read()
+def declared(header=HEADER, prefix="cable_gpu_"):
+    txt = open(header).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(cable_gpu_\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", txt)))
 
 
 def test_every_declared_function_is_exported(L):
@@ -35,6 +35,15 @@ def test_every_declared_function_is_exported(L):
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert sorted(gpu.EXPORTS) == names
+
+
+def test_surrogate_functions_are_exported(L):
+    from cable_amd import surrogate
+    names = declared(os.path.join(ROOT, "include", "cable_mlp.h"), "cable_mlp_")
+    assert len(names) >= 13
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert surrogate.lib().cable_mlp_nparam() == surrogate.NPARAM == 37256
 
 
 def test_field_table_matches_python_mirror(L):
