@@ -95,6 +95,18 @@ struct cable_gpu_ctx {
   // device position j holds the caller's tile order[j] (empty: identity)
   std::vector<int64_t> order;
   bool touched = false;             // device fields hold anything but zeros
+  // ---- surrogate (mixed contexts) ----
+  float *hyb = nullptr;             // [3][mp] previous-step leaf/canopy-air state
+  bool hyb_valid = false;
+  float *sx = nullptr, *sy = nullptr;  // captured training samples
+  unsigned long long *scount = nullptr;
+  int64_t scap = 0;
+  float *sparams = nullptr, *snorm = nullptr, *pred = nullptr;  // network, [mp][8]
+  uint16_t *swt = nullptr;          // transposed BF16 weights, per layer [out][in]
+  int *queue = nullptr;             // fallback tiles of a step
+  unsigned *qlen = nullptr;
+  unsigned long long *qtotal = nullptr;
+  int64_t last_fallbacks = 0, last_tile_steps = 0;
 };
 
 namespace {
@@ -223,6 +235,10 @@ int cable_gpu_destroy(cable_gpu_ctx *c) {
   if (c->d_vtab) hipFree(c->d_vtab);
   if (c->d_stab) hipFree(c->d_stab);
   if (c->d_npass) hipFree(c->d_npass);
+  void *sur[] = {c->hyb, c->sx, c->sy, c->scount, c->sparams, c->snorm, c->pred, c->swt, c->queue, c->qlen,
+                 c->qtotal};
+  for (void *p : sur)
+    if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -391,6 +407,7 @@ int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
     return fail(CABLE_E_ARG, "cable_gpu_run: steps %d..%d outside the %d resident forcing steps", k0,
                 k0 + nsteps - 1, c->nsteps_forc);
   HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
+  c->hyb_valid = false;   // the plain step does not keep the surrogate's inputs
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   for (int k = k0; k < k0 + nsteps; k++) {
     StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
@@ -406,6 +423,166 @@ int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
   unsigned long long np = 0;
   HIPCHECK(hipMemcpy(&np, c->d_npass, sizeof np, hipMemcpyDeviceToHost));
   c->last_passes = (int64_t)np;
+  return CABLE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// surrogate: capture, weights, hybrid run
+
+namespace {
+int check_resident(cable_gpu_ctx *c, int k0, int nsteps, double dels, const char *who) {
+  if (!c) return fail(CABLE_E_ARG, "%s: null context", who);
+  if (c->precision != CABLE_PREC_MIXED) return fail(CABLE_E_STATE, "%s: needs a mixed-precision context", who);
+  if (!c->have_tables || c->nsteps_forc == 0)
+    return fail(CABLE_E_STATE, "%s: class tables and forcing must be set first", who);
+  if (k0 < 0 || nsteps <= 0 || k0 + nsteps > c->nsteps_forc || !(dels > 0))
+    return fail(CABLE_E_ARG, "%s: steps %d..%d outside the %d resident forcing steps", who, k0, k0 + nsteps - 1,
+                c->nsteps_forc);
+  return CABLE_OK;
+}
+
+int ensure_hyb(cable_gpu_ctx *c) {
+  if (!c->hyb) {
+    HIPCHECK(hipMalloc(&c->hyb, sizeof(float) * 3 * (size_t)c->mp));
+    c->hyb_valid = false;
+  }
+  return CABLE_OK;
+}
+}  // namespace
+
+int cable_gpu_run_capture(cable_gpu_ctx *c, int k0, int nsteps, double dels, int64_t capacity) {
+  int r = check_resident(c, k0, nsteps, dels, "cable_gpu_run_capture");
+  if (r) return r;
+  if (capacity <= 0) return fail(CABLE_E_ARG, "cable_gpu_run_capture: capacity %lld", (long long)capacity);
+  if ((r = ensure_hyb(c))) return r;
+  if (capacity > c->scap) {
+    if (c->sx) HIPCHECK(hipFree(c->sx));
+    if (c->sy) HIPCHECK(hipFree(c->sy));
+    c->sx = c->sy = nullptr;
+    c->scap = 0;
+    HIPCHECK(hipMalloc(&c->sx, sizeof(float) * cbm_mlp::NIN * (size_t)capacity));
+    HIPCHECK(hipMalloc(&c->sy, sizeof(float) * cbm_mlp::NOUT * (size_t)capacity));
+    c->scap = capacity;
+  }
+  if (!c->scount) HIPCHECK(hipMalloc(&c->scount, sizeof(unsigned long long)));
+  HIPCHECK(hipMemsetAsync(c->scount, 0, sizeof(unsigned long long), c->stream));
+  HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
+  for (int k = k0; k < k0 + nsteps; k++) {
+    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
+    a.hyb = c->hyb;
+    a.hyb_valid = c->hyb_valid ? 1 : 0;
+    a.sx = c->sx;
+    a.sy = c->sy;
+    a.scount = c->scount;
+    a.scap = (unsigned long long)c->scap;
+    hipError_t e = launch_mixed_mode(MODE_CAPTURE, a, c->mp, c->stream);
+    if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run_capture launch: %s", hipGetErrorString(e));
+    c->first_call = false;
+    c->hyb_valid = true;
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  unsigned long long np = 0;
+  HIPCHECK(hipMemcpy(&np, c->d_npass, sizeof np, hipMemcpyDeviceToHost));
+  c->last_passes = (int64_t)np;
+  return CABLE_OK;
+}
+
+int cable_gpu_samples(cable_gpu_ctx *c, const float **x, const float **y, int64_t *n) {
+  if (!c || !x || !y || !n) return fail(CABLE_E_ARG, "cable_gpu_samples: null argument");
+  if (!c->scount) return fail(CABLE_E_STATE, "cable_gpu_samples: nothing captured");
+  unsigned long long k = 0;
+  HIPCHECK(hipMemcpy(&k, c->scount, sizeof k, hipMemcpyDeviceToHost));
+  *x = c->sx;
+  *y = c->sy;
+  *n = (int64_t)(k < (unsigned long long)c->scap ? k : (unsigned long long)c->scap);
+  return CABLE_OK;
+}
+
+int cable_gpu_copy_samples(cable_gpu_ctx *c, int64_t first, int64_t n, float *x, float *y) {
+  if (!c || !x || !y || first < 0 || n <= 0) return fail(CABLE_E_ARG, "cable_gpu_copy_samples: bad arguments");
+  if (first + n > c->scap) return fail(CABLE_E_ARG, "cable_gpu_copy_samples: rows beyond the capture buffer");
+  HIPCHECK(hipMemcpy(x, c->sx + first * cbm_mlp::NIN, sizeof(float) * cbm_mlp::NIN * (size_t)n, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(y, c->sy + first * cbm_mlp::NOUT, sizeof(float) * cbm_mlp::NOUT * (size_t)n, hipMemcpyDeviceToHost));
+  return CABLE_OK;
+}
+
+int cable_gpu_set_surrogate(cable_gpu_ctx *c, const float *params, const float *norm) {
+  using namespace cbm_mlp;
+  if (!c || !params || !norm) return fail(CABLE_E_ARG, "cable_gpu_set_surrogate: null argument");
+  for (int k = 0; k < NIN; k++)
+    if (!(norm[NIN + k] > 0)) return fail(CABLE_E_ARG, "cable_gpu_set_surrogate: sd_x[%d] not positive", k);
+  // weights transposed to [out][in] and rounded to BF16 for the matrix cores
+  std::vector<uint16_t> wt;
+  for (int l = 0; l < 4; l++)
+    for (int j = 0; j < LAYER_OUT[l]; j++)
+      for (int i = 0; i < LAYER_IN[l]; i++) wt.push_back(f32_to_bf16(params[LAYER_W[l] + i * LAYER_OUT[l] + j]));
+  if (!c->sparams) HIPCHECK(hipMalloc(&c->sparams, sizeof(float) * NPARAM));
+  if (!c->snorm) HIPCHECK(hipMalloc(&c->snorm, sizeof(float) * NNORM));
+  if (!c->swt) HIPCHECK(hipMalloc(&c->swt, sizeof(uint16_t) * wt.size()));
+  HIPCHECK(hipMemcpy(c->sparams, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->snorm, norm, sizeof(float) * NNORM, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->swt, wt.data(), sizeof(uint16_t) * wt.size(), hipMemcpyHostToDevice));
+  return CABLE_OK;
+}
+
+int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int max_passes, double resid_max,
+                         int use_surrogate) {
+  int r = check_resident(c, k0, nsteps, dels, "cable_gpu_run_hybrid");
+  if (r) return r;
+  if (use_surrogate && !c->swt) return fail(CABLE_E_STATE, "cable_gpu_run_hybrid: no surrogate set");
+  if (max_passes < 1 || max_passes > 20) return fail(CABLE_E_ARG, "cable_gpu_run_hybrid: max_passes %d", max_passes);
+  if ((r = ensure_hyb(c))) return r;
+  if (!c->pred) HIPCHECK(hipMalloc(&c->pred, sizeof(float) * cbm_mlp::NOUT * (size_t)c->mp));
+  if (!c->queue) HIPCHECK(hipMalloc(&c->queue, sizeof(int) * (size_t)c->mp));
+  if (!c->qlen) HIPCHECK(hipMalloc(&c->qlen, sizeof(unsigned)));
+  if (!c->qtotal) HIPCHECK(hipMalloc(&c->qtotal, sizeof(unsigned long long)));
+  HIPCHECK(hipMemsetAsync(c->qtotal, 0, sizeof(unsigned long long), c->stream));
+  HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  for (int k = k0; k < k0 + nsteps; k++) {
+    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
+    a.hyb = c->hyb;
+    a.hyb_valid = c->hyb_valid ? 1 : 0;
+    a.norm = c->snorm;
+    a.pred = use_surrogate ? c->pred : nullptr;
+    a.max_passes = max_passes;
+    a.resid_max = (float)resid_max;
+    a.queue = c->queue;
+    a.qlen = c->qlen;
+    HIPCHECK(hipMemsetAsync(c->qlen, 0, sizeof(unsigned), c->stream));
+    hipError_t e = hipSuccess;
+    if (use_surrogate) e = launch_mlp_infer(a, c->swt, c->sparams, c->pred, c->mp, c->stream);
+    if (e == hipSuccess) e = launch_mixed_mode(MODE_HYBRID, a, c->mp, c->stream);
+    // the tiles the capped iteration did not settle: the plain method, in
+    // place, from the state the hybrid step left untouched for them
+    StepArgs f = a;
+    f.tile_index = c->queue;
+    f.ntiles = c->qlen;
+    f.pred = nullptr;
+    if (e == hipSuccess) e = launch_mixed_mode(MODE_FALLBACK, f, c->mp, c->stream);
+    if (e == hipSuccess) e = launch_count(c->qlen, c->qtotal, c->stream);
+    if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run_hybrid launch: %s", hipGetErrorString(e));
+    c->first_call = false;
+    c->hyb_valid = true;
+  }
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  HIPCHECK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  unsigned long long np = 0, fb = 0;
+  HIPCHECK(hipMemcpy(&np, c->d_npass, sizeof np, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(&fb, c->qtotal, sizeof fb, hipMemcpyDeviceToHost));
+  c->last_passes = (int64_t)np;
+  c->last_fallbacks = (int64_t)fb;
+  c->last_tile_steps = (int64_t)nsteps * c->mp;
+  return CABLE_OK;
+}
+
+int cable_gpu_hybrid_counters(cable_gpu_ctx *c, int64_t *fallback_tile_steps, int64_t *tile_steps) {
+  if (!c || !fallback_tile_steps || !tile_steps) return fail(CABLE_E_ARG, "cable_gpu_hybrid_counters: null argument");
+  *fallback_tile_steps = c->last_fallbacks;
+  *tile_steps = c->last_tile_steps;
   return CABLE_OK;
 }
 

@@ -14,8 +14,15 @@
 #include <hip/hip_runtime.h>
 
 #include "cbm_step.h"
+#include "mlp_kernels.h"
 
 namespace cbm {
+
+namespace mlp = cbm_mlp;
+using cbm_mlp::f32_to_bf16;
+using cbm_mlp::silu;
+using cbm_mlp::v16f;
+using cbm_mlp::v4s;
 
 enum { ROLE_PARAM = 0, ROLE_FORC = 1, ROLE_STATE = 2, ROLE_DIAG = 3 };
 
@@ -41,20 +48,103 @@ constexpr int role_id(int r) { return r; }
 // FP32 live set fits it and the extra wave hides the latency of the LDS
 // table reads and transcendental sequences); FP64 keeps one wave per SIMD,
 // which measured faster than spilling its larger live set
-template <class R, bool FULL, bool TABLES>
+// ---------------------------------------------------------------------------
+// surrogate inputs: 24 features of a tile at the start of a step, read from
+// the device arrays (the resident run's layout: forcing at a.foff) and the
+// class-table rows.  The capture step and the inference kernel both call
+// this, so training and use see the same features.
+template <class R>
+__device__ inline void surrogate_features(const StepArgs &a, int64_t t, const double *vrow, const double *srow,
+                                          float *x) {
+  using DT = typename DevT<R, 'F'>::type;
+  void *const *F = a.F;
+  const int64_t mp = a.mp, f = t + a.foff;
+  const float tk = (float)ld<DT>(F[CABLE_F_met_tk], f), qv = (float)ld<DT>(F[CABLE_F_met_qv], f);
+  const int iveg = ld<int>(F[CABLE_F_veg_iveg], t);
+  const float vlai = iveg >= 14 ? 0.0f : (float)ld<DT>(F[CABLE_F_veg_vlai], t);
+  x[0] = (float)ld<DT>(F[CABLE_F_met_fsd], f);
+  x[1] = (float)ld<DT>(F[CABLE_F_met_fld], f);
+  x[2] = tk - 273.16f;
+  x[3] = qv * 1.0e3f;
+  x[4] = (float)ld<DT>(F[CABLE_F_met_ua], f);
+  x[5] = (float)ld<DT>(F[CABLE_F_met_pmb], f);
+  x[6] = (float)ld<DT>(F[CABLE_F_met_coszen], f);
+  x[7] = vlai;
+  x[8] = (float)vrow[VC_vcmax] * 1.0e6f;
+  x[9] = (float)vrow[VC_frac4];
+  x[10] = (float)vrow[VC_hc];
+  x[11] = (float)vrow[VC_a1gs];
+  x[12] = (float)vrow[VC_d0gs];
+  x[13] = (float)vrow[VC_dleaf];
+  x[14] = (float)vrow[VC_xfang];
+  x[15] = (float)vrow[VC_extkn];
+  // root-zone water stress of the standard scheme (cbl_fwsoil.F90:13-38)
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < MS; k++) {
+    const double wl = ld<double>(F[CABLE_F_ssnow_wb], k * mp + t) - ld<double>(F[CABLE_F_ssnow_wbice], k * mp + t);
+    const double sw = srow[SC_swilt_vec + k], fc = srow[SC_sfc_vec + k];
+    float r = (float)((wl - sw) / (fc - sw));
+    r = r < 1.0e-9f ? 1.0e-9f : (r > 1.0f ? 1.0f : r);
+    s += (float)vrow[VC_froot + k] * r;
+  }
+  float fw = (float)vrow[VC_vbeta] * (s > 1.0e-9f ? s : 1.0e-9f);
+  x[16] = fw < 1.0e-9f ? 1.0e-9f : (fw > 1.0f ? 1.0f : fw);
+  const bool h = a.hyb_valid && a.hyb;
+  x[17] = (h ? a.hyb[t] : tk) - tk;
+  x[18] = (h ? a.hyb[mp + t] : tk) - tk;
+  x[19] = ((h ? a.hyb[2 * mp + t] : qv) - qv) * 1.0e3f;
+  x[20] = (float)ld<DT>(F[CABLE_F_ssnow_tss], t) - tk;
+  x[21] = (float)ld<DT>(F[CABLE_F_ssnow_snowd], t);
+  x[22] = (float)ld<DT>(F[CABLE_F_canopy_cansto], t);
+  x[23] = (float)ld<DT>(F[CABLE_F_ssnow_rtsoil], t);
+}
+
+// wave-wide append: lanes with `take` get consecutive slots from *count
+__device__ inline unsigned long long wave_append(bool take, unsigned long long *count) {
+  const unsigned long long mask = __ballot(take);
+  const int lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  if (lane == 0 && mask) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+  base = __shfl(base, 0, 64);
+  return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+}
+__device__ inline unsigned wave_append32(bool take, unsigned *count) {
+  const unsigned long long mask = __ballot(take);
+  const int lane = threadIdx.x & 63;
+  unsigned base = 0;
+  if (lane == 0 && mask) base = atomicAdd(count, (unsigned)__popcll(mask));
+  base = __shfl(base, 0, 64);
+  return base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+template <class R, bool FULL, bool TABLES, int MODE = MODE_PLAIN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 2 : 1)))
 cbm_kernel(StepArgs a) {
   __shared__ double lds_v[TABLES ? NVEGCLASS * VC_N : 1];
   __shared__ double lds_s[TABLES ? NSOILCLASS * SC_N : 1];
+  const int64_t mp = a.mp;
+  // a fallback launch covers mp threads but steps only *a.ntiles tiles:
+  // workgroups past the end leave (all their threads, before any barrier)
+  if (a.tile_index && (int64_t)blockIdx.x * blockDim.x >= (int64_t)*a.ntiles) return;
   if constexpr (TABLES) {
     for (int i = threadIdx.x; i < NVEGCLASS * VC_N; i += blockDim.x) lds_v[i] = a.vtab[i];
     for (int i = threadIdx.x; i < NSOILCLASS * SC_N; i += blockDim.x) lds_s[i] = a.stab[i];
     __syncthreads();
   }
-  const int64_t mp = a.mp;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // thread -> tile: direct, or through a.tile_index (the hybrid's fallback)
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = gi;
+  bool live = gi < mp;
+  if (a.tile_index) {
+    live = gi < (int64_t)*a.ntiles;
+    t = live ? (int64_t)a.tile_index[gi] : 0;
+  }
   int passes = 0;
-  if (t < mp) {
+  bool fallback = false, emit = false;
+  float feat[MODE == MODE_CAPTURE ? mlp::NIN : 1];
+  float tgt[MODE == MODE_CAPTURE ? mlp::NOUT : 1];
+  if (live) {
     void *const *F = a.F;
     Tile<R> T;
     // ---- gather ----
@@ -132,11 +222,33 @@ cbm_kernel(StepArgs a) {
     const int iv = T.veg_iveg[0] - 1, is = T.soil_isoilm[0] - 1;
     const double *vrow = TABLES ? lds_v + (iv < 0 ? 0 : iv) * VC_N : nullptr;
     const double *srow = TABLES ? lds_s + (is < 0 ? 0 : is) * SC_N : nullptr;
-    Stepper<R, FULL, TABLES> S(T, a, vrow, srow);
+    Stepper<R, FULL, TABLES, MODE> S(T, a, vrow, srow);
+    if constexpr (MODE == MODE_CAPTURE) surrogate_features<R>(a, t, vrow, srow, feat);
+    if constexpr (MODE == MODE_HYBRID) {
+      if (a.pred) {
+#pragma unroll
+        for (int c = 0; c < mlp::NOUT; c++) S.pred[c] = a.pred[t * mlp::NOUT + c];
+      }
+    }
     S.run();
     passes = S.npass;
-    // ---- scatter ----
-    if constexpr (FULL) {
+    if constexpr (MODE == MODE_HYBRID) fallback = S.vegetated && S.resid > a.resid_max;
+    if constexpr (MODE == MODE_CAPTURE) {
+      emit = S.vegetated;
+#pragma unroll
+      for (int c = 0; c < mlp::NOUT; c++) tgt[c] = S.tgt[c];
+    }
+    // the previous-step canopy state the surrogate's features read
+    if constexpr (MODE != MODE_PLAIN) {
+      if (a.hyb && !fallback) {
+        a.hyb[t] = (float)S.leaf_temperature;
+        a.hyb[mp + t] = (float)T.met_tvair[0];
+        a.hyb[2 * mp + t] = (float)T.met_qvair[0];
+      }
+    }
+    // ---- scatter (a tile that falls back keeps its old state) ----
+    if (fallback) {
+    } else if constexpr (FULL) {
 #define PARAM ROLE_PARAM
 #define FORC ROLE_FORC
 #define STATE ROLE_STATE
@@ -183,20 +295,136 @@ cbm_kernel(StepArgs a) {
     for (int o = 32; o > 0; o >>= 1) passes += __shfl_xor(passes, o, 64);
     if ((threadIdx.x & 63) == 0 && passes) atomicAdd(a.npass, (unsigned long long)passes);
   }
+  if constexpr (MODE == MODE_HYBRID) {
+    const unsigned slot = wave_append32(fallback, a.qlen);
+    if (fallback) a.queue[slot] = (int)t;   // the queue holds mp entries
+  }
+  if constexpr (MODE == MODE_CAPTURE) {
+    const unsigned long long slot = wave_append(emit, a.scount);
+    if (emit && slot < a.scap) {
+#pragma unroll
+      for (int c = 0; c < mlp::NIN; c++) a.sx[slot * mlp::NIN + c] = feat[c];
+#pragma unroll
+      for (int c = 0; c < mlp::NOUT; c++) a.sy[slot * mlp::NOUT + c] = tgt[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// surrogate inference for every tile of a step: 24 features -> network ->
+// 8 denormalised predictions in pred[t*8 ..].  Four waves per workgroup, 32
+// tiles per wave; activations in LDS as BF16, weights read in their
+// transposed BF16 layout wt (per layer [out][in]) so each lane's four
+// K-consecutive operands are one 8-byte load; biases FP32 from `params`.
+template <int NWAVES>
+__global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, const uint16_t *__restrict__ wt,
+                                                        const float *__restrict__ params,
+                                                        float *__restrict__ pred) {
+  using namespace mlp;
+  // rows padded by 4 and 8 BF16 so that the 32 rows a wave reads at one K
+  // offset fall in different LDS banks
+  __shared__ uint16_t xs[NWAVES][32][NIN + 4];
+  __shared__ uint16_t hs[NWAVES][32][HID + 4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = lane & 31, kq = 4 * (lane >> 5);
+  const int64_t base = ((int64_t)blockIdx.x * NWAVES + wave) * 32;
+  const float *norm = a.norm;
+  if (lane < 32) {
+    const int64_t t = base + lane;
+    float x[NIN];
+    if (t < a.mp) {
+      const int iv = ld<int>(a.F[CABLE_F_veg_iveg], t) - 1, is = ld<int>(a.F[CABLE_F_soil_isoilm], t) - 1;
+      surrogate_features<float>(a, t, a.vtab + (iv < 0 ? 0 : iv) * VC_N, a.stab + (is < 0 ? 0 : is) * SC_N, x);
+#pragma unroll
+      for (int c = 0; c < NIN; c++) x[c] = (x[c] - norm[c]) / norm[NIN + c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < NIN; c++) x[c] = 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < NIN; c++) xs[wave][lane][c] = f32_to_bf16(x[c]);
+  }
+  __syncthreads();
+  // hidden layers
+  int woff = 0;
+  for (int l = 0; l < 3; l++) {
+    const int K = LAYER_IN[l];
+    const uint16_t *W = wt + woff;
+    const float *bias = params + LAYER_B[l];
+    v16f acc[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[nb][r] = 0.0f;
+      for (int k0 = 0; k0 < K; k0 += 8) {
+        v4s av, bv;
+        const uint16_t *src = (l == 0) ? &xs[wave][row][k0 + kq] : &hs[wave][row][k0 + kq];
+        const uint16_t *wsrc = W + (int64_t)(nb * 32 + row) * K + k0 + kq;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          av[e] = (short)src[e];
+          bv[e] = (short)wsrc[e];
+        }
+        acc[nb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc[nb], 0, 0, 0);
+      }
+    }
+    __syncthreads();  // every wave has read this layer's input
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const int j = nb * 32 + row;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int i = (r & 3) + kq + 8 * (r >> 2);
+        hs[wave][i][j] = f32_to_bf16(silu(acc[nb][r] + bias[j]));
+      }
+    }
+    __syncthreads();
+    woff += K * HID;
+  }
+  // output layer: N = 8 of the 32 columns
+  {
+    const uint16_t *W = wt + woff;
+    const float *bias = params + LAYER_B[3];
+    v16f acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    for (int k0 = 0; k0 < HID; k0 += 8) {
+      v4s av, bv;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        av[e] = (short)hs[wave][row][k0 + kq + e];
+        bv[e] = row < NOUT ? (short)W[row * HID + k0 + kq + e] : (short)0;
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    if (row < NOUT) {
+      const float mu = norm[2 * NIN + row], sd = norm[2 * NIN + NOUT + row];
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int64_t t = base + (r & 3) + kq + 8 * (r >> 2);
+        if (t < a.mp) pred[t * NOUT + row] = (acc[r] + bias[row]) * sd + mu;
+      }
+    }
+  }
 }
 
 
 // launch one step over mp tiles on `stream` (256 threads per workgroup)
-template <class R, bool FULL, bool TABLES>
+template <class R, bool FULL, bool TABLES, int MODE = MODE_PLAIN>
 hipError_t launch_cbm(const StepArgs &a, int64_t mp, hipStream_t stream) {
   const int block = 256;
   const int64_t grid = (mp + block - 1) / block;
-  hipLaunchKernelGGL((cbm_kernel<R, FULL, TABLES>), dim3((unsigned)grid), dim3(block), 0, stream, a);
+  hipLaunchKernelGGL((cbm_kernel<R, FULL, TABLES, MODE>), dim3((unsigned)grid), dim3(block), 0, stream, a);
   return hipGetLastError();
 }
 
 // defined in cbm_kernels_fp64.hip and cbm_kernels_mixed.hip
 hipError_t launch_fp64(bool full, const StepArgs &a, int64_t mp, hipStream_t stream);
 hipError_t launch_mixed(bool full, const StepArgs &a, int64_t mp, hipStream_t stream);
+// mixed path, resident layout: a step in MODE_CAPTURE or MODE_HYBRID, and the
+// surrogate inference over mp tiles (cbm_kernels_mixed.hip)
+hipError_t launch_mixed_mode(int mode, const StepArgs &a, int64_t mp, hipStream_t stream);
+hipError_t launch_mlp_infer(const StepArgs &a, const uint16_t *wt, const float *params, float *pred,
+                            int64_t mp, hipStream_t stream);
+hipError_t launch_count(const unsigned *qlen, unsigned long long *total, hipStream_t stream);
 
 }  // namespace cbm

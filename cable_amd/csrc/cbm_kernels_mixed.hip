@@ -10,4 +10,28 @@ hipError_t launch_mixed(bool full, const StepArgs &a, int64_t mp, hipStream_t st
   return full ? launch_cbm<float, true, false>(a, mp, stream) : launch_cbm<float, false, true>(a, mp, stream);
 }
 
+hipError_t launch_mixed_mode(int mode, const StepArgs &a, int64_t mp, hipStream_t stream) {
+  switch (mode) {
+    case MODE_CAPTURE: return launch_cbm<float, false, true, MODE_CAPTURE>(a, mp, stream);
+    case MODE_HYBRID: return launch_cbm<float, false, true, MODE_HYBRID>(a, mp, stream);
+    case MODE_FALLBACK: return launch_cbm<float, false, true, MODE_FALLBACK>(a, mp, stream);
+    default: return launch_cbm<float, false, true, MODE_PLAIN>(a, mp, stream);
+  }
+}
+
+// total of the per-step fallback counts (one thread)
+__global__ void count_kernel(const unsigned *qlen, unsigned long long *total) { *total += *qlen; }
+
+hipError_t launch_count(const unsigned *qlen, unsigned long long *total, hipStream_t stream) {
+  hipLaunchKernelGGL(count_kernel, dim3(1), dim3(1), 0, stream, qlen, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer(const StepArgs &a, const uint16_t *wt, const float *params, float *pred, int64_t mp,
+                            hipStream_t stream) {
+  const int64_t grid = (mp + 127) / 128;   // 4 waves x 32 tiles per workgroup
+  hipLaunchKernelGGL(mlp_infer_kernel<4>, dim3((unsigned)grid), dim3(256), 0, stream, a, wt, params, pred);
+  return hipGetLastError();
+}
+
 }  // namespace cbm

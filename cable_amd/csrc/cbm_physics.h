@@ -107,6 +107,15 @@ struct Tile {
 #undef X
 };
 
+// what a step does besides the physics
+enum StepMode {
+  MODE_PLAIN = 0,    // the reference's method
+  MODE_CAPTURE = 1,  // the reference's method, and write surrogate training samples
+  MODE_HYBRID = 2,   // surrogate warm start, capped polish passes, fallback flag
+  MODE_FALLBACK = 3  // the reference's method on the hybrid's fallback tiles,
+                     // keeping the surrogate's inputs for the next step
+};
+
 // globals of one step
 struct StepArgs {
   void *const *F;        // device pointer table: F[field id] -> (n, mp) array
@@ -124,6 +133,21 @@ struct StepArgs {
                          // precip_sn from tk, ca and doy scalars
   double ca, doy;
   unsigned long long *npass;  // optional: sum of dryLeaf passes (work counter)
+  // ---- surrogate (mixed path only) ----
+  float *hyb;                // [3][mp]: previous step's leaf and canopy-air
+                             // temperature, canopy-air humidity (or null)
+  int hyb_valid;             // 0: hyb not yet written (first step)
+  const float *norm;         // feature/target normalisation (64 values)
+  const float *pred;         // [mp][8] surrogate predictions (MODE_HYBRID)
+  float *sx, *sy;            // [cap][24], [cap][8] training samples (MODE_CAPTURE)
+  unsigned long long *scount;  // samples written so far
+  unsigned long long scap;     // capacity of sx/sy
+  int max_passes;            // MODE_HYBRID: dryLeaf passes per NITER pass
+  float resid_max;           // MODE_HYBRID: leaf energy residual limit, W/m2
+  int *queue;                // MODE_HYBRID: tiles that fall back
+  unsigned *qlen;            // entries in queue
+  const int *tile_index;     // fallback launch: the tiles to step (or null)
+  const unsigned *ntiles;    // fallback launch: number of entries in tile_index
 };
 
 // literal of the working precision

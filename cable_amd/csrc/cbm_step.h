@@ -1,19 +1,31 @@
 // cbm_step.h -- the per-tile CABLE timestep on the GPU (device code).
 //
-// Stepper<R, FULL, TABLES>::run() performs, for the tile held in `t`, one
-// call of the reference's cbm() together with the offline driver's contract
-// around it; see cbm_physics.h for the template parameters and layout.
+// Stepper<R, FULL, TABLES, MODE>::run() performs, for the tile held in `t`,
+// one call of the reference's cbm() together with the offline driver's
+// contract around it; see cbm_physics.h for the template parameters and
+// layout.  MODE (StepMode) adds the surrogate's work: MODE_CAPTURE records
+// the converged leaf state as a training target; MODE_HYBRID starts the leaf
+// fixed point from the surrogate's prediction, caps its passes and reports
+// whether the tile must fall back to the plain method.
 #pragma once
 #include "cbm_physics.h"
 
 namespace cbm {
 
-template <class R, bool FULL, bool TABLES>
+template <class R, bool FULL, bool TABLES, int MODE = MODE_PLAIN>
 struct Stepper {
   Tile<R> &t;
   const StepArgs &a;
   const double *vrow, *srow;
   int npass = 0;
+  // surrogate: predictions in (MODE_HYBRID), targets out (MODE_CAPTURE), the
+  // largest leaf energy-balance residual left by the capped passes and the
+  // resulting fallback decision (MODE_HYBRID)
+  float pred[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  float tgt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  float resid = 0.0f;
+  bool vegetated = false;
+  R leaf_temperature = R(0);   // the leaf fixed point's final temperature
 
   // ---- constants: src/params/*, src/util/cable_common.F90:217-222 ----
   static constexpr R TFRZ = R(273.16), SBOLTZ = R(5.67e-8), EMSOIL = R(1.0),
@@ -551,7 +563,10 @@ struct Stepper {
     const R tvair = t.met_tvair[0], tk = t.met_tk[0];
     const R fvl0 = t.rad_fvlai[0], fvl1 = t.rad_fvlai[1];
     const bool leaves = fvl0 + fvl1 > LAI_THRESH;
-    for (int k = 1; k <= 20; k++) {
+    // the reference's fixed count of 20 passes; the hybrid polishes the
+    // surrogate's start with at most a.max_passes
+    const int kmax = (MODE == MODE_HYBRID) ? a.max_passes : 20;
+    for (int k = 1; k <= kmax; k++) {
       bool active = (vlaiw > LAI_THRESH && abs_deltlf > K(0.1));
       // a tile that has converged keeps every value: the remaining passes of
       // the reference's fixed-count loop change nothing for it
@@ -673,6 +688,16 @@ struct Stepper {
         an_y[1] = anx[1];
 #pragma unroll
         for (int q = 0; q < MS; q++) oldevapfbl[q] = t.ssnow_evapfbl[q];
+      }
+    }
+    if constexpr (MODE == MODE_HYBRID) {
+      // leaf energy-balance residual at the last pass's starting temperature:
+      // the leaf sensible-plus-radiative conductance times the temperature
+      // change that pass still made (cr * sum(ghr) * |dT|), counted beyond
+      // the 0.1 K the reference's own convergence test accepts
+      if (vlaiw > LAI_THRESH && abs_deltlf > K(0.1)) {
+        const float r = (float)(CAPP * RMAIR * (ghr[0] + ghr[1]) * (abs_deltlf - K(0.1)));
+        resid = r > resid ? r : resid;
       }
     }
     R fwet = t.canopy_fwet[0];
@@ -868,6 +893,23 @@ struct Stepper {
     c.tlfx = t.met_tk[0];
     c.tlfy = t.met_tk[0];
     c.fwsoil = K(1.0);
+    // (the plain method's instantiation keeps no such state: stores to these
+    // members changed the FP64 kernels' results on gfx950 although nothing
+    // reads them, so MODE_PLAIN compiles them out)
+    if constexpr (MODE != MODE_PLAIN) vegetated = veg;
+    if constexpr (MODE == MODE_HYBRID) {
+      // warm start of the leaf fixed point of the first NITER pass at the
+      // surrogate's prediction of where that pass converges: leaf
+      // temperature, the leaf-surface CO2 of both leaf classes, and the leaf
+      // vapour-pressure deficit that temperature implies (without
+      // predictions the iteration starts as the reference's does)
+      if (veg && a.pred) {
+        c.tlfx = t.met_tk[0] + (R)pred[0];
+        c.csx[0] = MAX(K(1.0e-4), t.met_ca[0] * (R)pred[1]);
+        c.csx[1] = MAX(K(1.0e-4), t.met_ca[0] * (R)pred[2]);
+        c.dsx = MAX(K(0.0), t.met_dva[0] + t.air_dsatdk[0] * (c.tlfx - t.met_tvair[0]));
+      }
+    }
     R ortsoil = t.ssnow_rtsoil[0];
     int isflag = t.ssnow_isflag[0];
     t.ssnow_tss[0] = (R)(1 - isflag) * t.ssnow_tgg[0] + (R)isflag * t.ssnow_tggsn[0];
@@ -927,6 +969,13 @@ struct Stepper {
       c.hcy = K(0.0);
       c.ecy = c.rny - c.hcy;
       dryLeaf(c, iter);
+      if constexpr (MODE == MODE_CAPTURE) {
+        if (iter == 1) {
+          tgt[0] = (float)(c.tlfx - t.met_tk[0]);
+          tgt[1] = (float)(c.csx[0] / t.met_ca[0]);
+          tgt[2] = (float)(c.csx[1] / t.met_ca[0]);
+        }
+      }
       wetLeaf(c);
       t.canopy_fev[0] = (R)(t.canopy_fevc[0] + (double)t.canopy_fevw[0]);
       R fwet = t.canopy_fwet[0];
@@ -1001,6 +1050,17 @@ struct Stepper {
     t.canopy_zetar[1] = zeta1;
     t.canopy_zetar[2] = zeta2;
     t.canopy_zetar[3] = zeta3;
+    if constexpr (MODE == MODE_CAPTURE) {
+      // targets 0-2 (above): where the first NITER pass's leaf fixed point
+      // settles; 3-7: the step's final leaf temperature, stomatal
+      // conductances of both leaf classes, canopy latent and sensible heat
+      tgt[3] = (float)(c.tlfx - t.met_tk[0]);
+      tgt[4] = (float)t.canopy_gswx[0];
+      tgt[5] = (float)t.canopy_gswx[1];
+      tgt[6] = (float)c.ecy;
+      tgt[7] = (float)c.hcy;
+    }
+    if constexpr (MODE != MODE_PLAIN) leaf_temperature = c.tlfx;
     R us = t.canopy_us[0], ua = t.met_ua[0];
     R rho = t.air_rho[0];
     R rlam = t.air_rlam[0];

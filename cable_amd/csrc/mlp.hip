@@ -171,7 +171,10 @@ struct cable_mlp {
   float *xs = nullptr, *ys = nullptr;
   int64_t nsamples = 0;
   bool own_samples = false;
-  int64_t *idx = nullptr;
+  int64_t *idx = nullptr;     // sample order of the batches being trained
+  int64_t idxcap = 0;
+  float *losses = nullptr;    // per-batch losses of the current train call
+  int64_t losscap = 0;
   int64_t adam_t = 0;
 };
 
@@ -207,8 +210,7 @@ int forward(cable_mlp *m, int n) {
 }
 
 // gradients of the batch loss w.r.t. every parameter, into m->g
-int backward(cable_mlp *m, int n) {
-  float *loss = m->loss;
+int backward(cable_mlp *m, int n, float *loss) {
   HIPCHECK(hipMemsetAsync(loss, 0, sizeof(float), m->stream));
   hipLaunchKernelGGL(mse_kernel, dim3((n * NOUT + 255) / 256), dim3(256), 0, m->stream, m->out, m->y, n, m->dy,
                      loss);
@@ -285,7 +287,6 @@ int cable_mlp_create(cable_mlp **out, int batch) {
   al((void **)&m->dh, sizeof(float) * (size_t)batch * HID);
   al((void **)&m->dz, sizeof(float) * (size_t)batch * HID);
   al((void **)&m->loss, sizeof(float));
-  al((void **)&m->idx, sizeof(int64_t) * (size_t)batch);
   if (e != hipSuccess) {
     cable_mlp_destroy(m);
     return fail(CABLE_MLP_E_HIP, "cable_mlp_create: %s", hipGetErrorString(e));
@@ -302,7 +303,7 @@ int cable_mlp_create(cable_mlp **out, int batch) {
 int cable_mlp_destroy(cable_mlp *m) {
   if (!m) return CABLE_MLP_OK;
   void *ps[] = {m->w, m->g, m->m, m->v, m->wb, m->norm, m->x, m->y, m->z[0], m->z[1], m->z[2], m->h[0], m->h[1],
-                m->h[2], m->out, m->dy, m->dh, m->dz, m->loss, m->idx};
+                m->h[2], m->out, m->dy, m->dh, m->dz, m->loss, m->idx, m->losses};
   for (void *p : ps)
     if (p) hipFree(p);
   if (m->own_samples) {
@@ -366,7 +367,7 @@ int cable_mlp_gradient(cable_mlp *m, const float *x, const float *t, int n, floa
   int r = copy_rows_in(m, x, m->x, n, NIN);
   if (!r) r = copy_rows_in(m, t, m->y, n, NOUT);
   if (!r) r = forward(m, n);
-  if (!r) r = backward(m, n);
+  if (!r) r = backward(m, n, m->loss);
   if (r) return r;
   if (loss) HIPCHECK(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
   HIPCHECK(hipStreamSynchronize(m->stream));
@@ -407,23 +408,36 @@ int cable_mlp_train(cable_mlp *m, const int64_t *order, int64_t nbatches, float 
   if (!m || !order || nbatches <= 0) return fail(CABLE_MLP_E_ARG, "cable_mlp_train: bad arguments");
   if (!m->xs) return fail(CABLE_MLP_E_STATE, "cable_mlp_train: no samples loaded");
   const int B = m->batch;
+  const int64_t total = nbatches * B;
+  for (int64_t i = 0; i < total; i++)
+    if (order[i] < 0 || order[i] >= m->nsamples)
+      return fail(CABLE_MLP_E_ARG, "cable_mlp_train: sample index %lld out of range", (long long)order[i]);
+  if (total > m->idxcap) {
+    if (m->idx) HIPCHECK(hipFree(m->idx));
+    m->idx = nullptr;
+    m->idxcap = 0;
+    HIPCHECK(hipMalloc(&m->idx, sizeof(int64_t) * (size_t)total));
+    m->idxcap = total;
+  }
+  if (nbatches > m->losscap) {
+    if (m->losses) HIPCHECK(hipFree(m->losses));
+    m->losses = nullptr;
+    m->losscap = 0;
+    HIPCHECK(hipMalloc(&m->losses, sizeof(float) * (size_t)nbatches));
+    m->losscap = nbatches;
+  }
+  HIPCHECK(hipMemcpyAsync(m->idx, order, sizeof(int64_t) * (size_t)total, hipMemcpyHostToDevice, m->stream));
   for (int64_t b = 0; b < nbatches; b++) {
-    const int64_t *ix = order + b * B;
-    for (int i = 0; i < B; i++)
-      if (ix[i] < 0 || ix[i] >= m->nsamples)
-        return fail(CABLE_MLP_E_ARG, "cable_mlp_train: sample index %lld out of range", (long long)ix[i]);
-    HIPCHECK(hipMemcpyAsync(m->idx, ix, sizeof(int64_t) * B, hipMemcpyHostToDevice, m->stream));
-    hipLaunchKernelGGL(gather_kernel, dim3((B + 255) / 256), dim3(256), 0, m->stream, m->xs, m->ys, m->idx, B, m->x,
-                       m->y, m->norm);
+    hipLaunchKernelGGL(gather_kernel, dim3((B + 255) / 256), dim3(256), 0, m->stream, m->xs, m->ys, m->idx + b * B, B,
+                       m->x, m->y, m->norm);
     HIPCHECK(hipGetLastError());
     int r = forward(m, B);
-    if (!r) r = backward(m, B);
+    if (!r) r = backward(m, B, m->losses + b);
     if (!r) r = adam(m, lr);
     if (r) return r;
-    if (losses) HIPCHECK(hipMemcpyAsync(losses + b, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
-    // the index buffer is reused by the next batch
-    HIPCHECK(hipStreamSynchronize(m->stream));
   }
+  if (losses) HIPCHECK(hipMemcpyAsync(losses, m->losses, sizeof(float) * (size_t)nbatches, hipMemcpyDeviceToHost, m->stream));
+  HIPCHECK(hipStreamSynchronize(m->stream));
   return CABLE_MLP_OK;
 }
 

@@ -31,6 +31,8 @@ EXPORTS = [
     "cable_gpu_veg_cols", "cable_gpu_soil_cols", "cable_gpu_table_column",
     "cable_gpu_set_class_tables", "cable_gpu_set_forcing", "cable_gpu_run",
     "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters", "cable_gpu_set_tile_order",
+    "cable_gpu_run_capture", "cable_gpu_samples", "cable_gpu_copy_samples", "cable_gpu_set_surrogate",
+    "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters",
     "cable_gpu_last_error",
 ]
 
@@ -73,6 +75,12 @@ def lib():
         "cable_gpu_counters": ([vp, ctypes.POINTER(i64)], c_int),
         "cable_gpu_last_error": ([], ctypes.c_char_p),
         "cable_gpu_set_tile_order": ([vp, vp], c_int),
+        "cable_gpu_run_capture": ([vp, c_int, c_int, c_double, i64], c_int),
+        "cable_gpu_samples": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i64)], c_int),
+        "cable_gpu_copy_samples": ([vp, i64, i64, vp, vp], c_int),
+        "cable_gpu_set_surrogate": ([vp, vp, vp], c_int),
+        "cable_gpu_run_hybrid": ([vp, c_int, c_int, c_double, c_int, c_double, c_int], c_int),
+        "cable_gpu_hybrid_counters": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -213,6 +221,39 @@ class CableGPU:
 
     def run(self, k0: int, nsteps: int, dels: float):
         _check(lib().cable_gpu_run(self.h, int(k0), int(nsteps), float(dels)))
+
+    # -- surrogate (mixed contexts) --------------------------------------------
+    def run_capture(self, k0: int, nsteps: int, dels: float, capacity: int):
+        """Resident steps that also record one training sample per vegetated
+        tile-step; returns the number of samples held."""
+        _check(lib().cable_gpu_run_capture(self.h, int(k0), int(nsteps), float(dels), int(capacity)))
+        return self.samples()[2]
+
+    def samples(self):
+        """(x device pointer, y device pointer, n) of the captured samples."""
+        x, y, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int64()
+        _check(lib().cable_gpu_samples(self.h, ctypes.byref(x), ctypes.byref(y), ctypes.byref(n)))
+        return x.value, y.value, n.value
+
+    def copy_samples(self, first: int, n: int):
+        x = np.empty((n, 24), dtype=np.float32)
+        y = np.empty((n, 8), dtype=np.float32)
+        _check(lib().cable_gpu_copy_samples(self.h, int(first), int(n), _ptr(x), _ptr(y)))
+        return x, y
+
+    def set_surrogate(self, params, norm):
+        self._sur = (np.ascontiguousarray(params, dtype=np.float32), np.ascontiguousarray(norm, dtype=np.float32))
+        _check(lib().cable_gpu_set_surrogate(self.h, _ptr(self._sur[0]), _ptr(self._sur[1])))
+
+    def run_hybrid(self, k0: int, nsteps: int, dels: float, max_passes=2, resid_max=0.5, use_surrogate=True):
+        _check(lib().cable_gpu_run_hybrid(self.h, int(k0), int(nsteps), float(dels), int(max_passes),
+                                          float(resid_max), 1 if use_surrogate else 0))
+
+    def hybrid_counters(self):
+        """(fallback tile-steps, tile-steps) of the last hybrid run."""
+        fb, n = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().cable_gpu_hybrid_counters(self.h, ctypes.byref(fb), ctypes.byref(n)))
+        return fb.value, n.value
 
     def last_run_ms(self):
         v = ctypes.c_double()

@@ -166,3 +166,53 @@ class MLP:
                                          losses[done:].ctypes.data))
             done += k
         return losses
+
+
+def capture_samples(n_tiles=262144, nsteps=64, seed=None, order=True):
+    """Run the plain mixed-precision step on a fresh synthetic workload and
+    keep one training sample per vegetated tile-step on the GPU.  Returns
+    the context (which owns the samples) and the sample count."""
+    from . import gpu
+    from .workload import DEFAULT_SEED, Workload
+    wl = Workload(n_tiles, nsteps, seed=(DEFAULT_SEED + 1) if seed is None else seed)
+    o = gpu.class_order(wl.iveg, wl.isoilm) if order else None
+    g = gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=o)
+    n = g.run_capture(0, nsteps, wl.dels, n_tiles * nsteps)
+    return g, n
+
+
+def sample_stats(g, n, chunks=32, rows=16384):
+    """Normalisation statistics from evenly spaced blocks of the captured
+    samples (which are stored in step order)."""
+    xs, ys = [], []
+    rows = min(rows, max(1, n // chunks))
+    for c in range(chunks):
+        first = min(n - rows, (c * n) // chunks)
+        x, y = g.copy_samples(first, rows)
+        xs.append(x)
+        ys.append(y)
+    return norm_stats(np.concatenate(xs), np.concatenate(ys))
+
+
+def train_on_gpu(n_tiles=262144, nsteps=64, epochs=2.0, batch=2048, lr=2e-3, seed=0, log=None):
+    """Capture samples with the plain kernel and train the network on them
+    with the GPU kernels.  Returns (params, norm, info)."""
+    import time
+    t0 = time.perf_counter()
+    g, n = capture_samples(n_tiles, nsteps)
+    t1 = time.perf_counter()
+    norm = sample_stats(g, n)
+    net = MLP(batch=batch, params=init_params(seed), norm=norm)
+    xp, yp, _ = g.samples()
+    net.use_device_samples(xp, yp, n)
+    losses = net.train(epochs=epochs, lr=lr, seed=seed)
+    t2 = time.perf_counter()
+    params = net.params()
+    info = {"samples": int(n), "capture_s": t1 - t0, "train_s": t2 - t1, "batches": int(losses.size),
+            "loss_first": float(losses[:16].mean()), "loss_last": float(losses[-16:].mean())}
+    if log:
+        log(f"[surrogate] {n} samples captured in {t1 - t0:.1f}s; {losses.size} batches in {t2 - t1:.1f}s; "
+            f"loss {info['loss_first']:.4f} -> {info['loss_last']:.4f}")
+    net.close()
+    g.close()
+    return params, norm, info

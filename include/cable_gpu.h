@@ -144,6 +144,46 @@ int cable_gpu_last_run_ms(cable_gpu_ctx *ctx, double *ms);
    and the four stability iterations), a work counter. */
 int cable_gpu_counters(cable_gpu_ctx *ctx, int64_t *dryleaf_passes);
 
+/* ---- canopy surrogate (mixed-precision contexts; no counterpart in the
+   reference) -------------------------------------------------------------
+   The surrogate (include/cable_mlp.h) maps a tile's forcing, state and
+   parameters at the start of a step to the converged state of the leaf
+   fixed point (src/science/canopy/cbl_dryLeaf.F90:10-666).  Its 24 inputs:
+   fsd(1), fld, tk - 273.16, 1000 qv, ua, pmb, coszen, vlai, 1e6 vcmax,
+   frac4, hc, a1gs, d0gs, dleaf, xfang, extkn, fwsoil (standard scheme),
+   the previous step's leaf temperature, canopy-air temperature (both minus
+   tk) and 1000 x (canopy-air humidity - qv), tss - tk, snowd, cansto,
+   rtsoil.  Its 8 outputs: leaf temperature - tk after the first and the last
+   NITER pass, leaf-surface CO2 / ca of the sunlit and shaded leaves after
+   the first pass (which carries the net assimilation, csx = ca - 1.32 An /
+   gb), the final stomatal conductances of both leaves, and the canopy's
+   latent and sensible heat. */
+
+/* Run steps k0..k0+nsteps-1 as cable_gpu_run, and append one training
+   sample (24 inputs, 8 converged outputs) per vegetated tile-step to a
+   device buffer of `capacity` samples (reallocated if larger). */
+int cable_gpu_run_capture(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, int64_t capacity);
+/* device pointers to the captured samples (row-major [n][24] and [n][8]),
+   valid until the next capture or destroy */
+int cable_gpu_samples(cable_gpu_ctx *ctx, const float **x, const float **y, int64_t *n);
+/* copy captured samples first..first+n-1 to host arrays [n][24], [n][8] */
+int cable_gpu_copy_samples(cable_gpu_ctx *ctx, int64_t first, int64_t n, float *x, float *y);
+/* network parameters and normalisation for the hybrid step (layouts as in
+   include/cable_mlp.h) */
+int cable_gpu_set_surrogate(cable_gpu_ctx *ctx, const float *params, const float *norm);
+/* Run steps k0..k0+nsteps-1 with the hybrid method: per step the network
+   predicts each tile's leaf state, the leaf fixed point starts there and
+   runs at most max_passes passes per NITER pass (the reference runs up to
+   20), and every tile whose leaf energy-balance residual beyond the
+   reference's 0.1 K tolerance exceeds resid_max (W/m2) is stepped again
+   with the plain method from its unchanged state.  use_surrogate = 0 runs
+   the same capped method from the reference's own starting point (the
+   comparison without the network). */
+int cable_gpu_run_hybrid(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, int max_passes,
+                         double resid_max, int use_surrogate);
+/* fallback tile-steps and all tile-steps of the last hybrid run */
+int cable_gpu_hybrid_counters(cable_gpu_ctx *ctx, int64_t *fallback_tile_steps, int64_t *tile_steps);
+
 const char *cable_gpu_last_error(void);
 
 #ifdef __cplusplus
