@@ -4,8 +4,10 @@ A step is one cbm pass over every tile of the synthetic workload: the
 device-resident path (cable_gpu_run) with state, parameters, class tables and
 the run's forcing already in HBM when the timed region starts.  The headline
 line is the plain FP64 path on 262,144 tiles x 48 half-hourly steps; the other
-forms (mixed precision, and FP64 at 1,048,576 tiles, the size the mixed and
-surrogate forms are compared at) are reported beside it under "forms".
+forms -- FP64, mixed precision and the surrogate-warm-started hybrid at
+1,048,576 tiles, with the mixed and hybrid fluxes' distance from FP64 after
+the run and the surrogate's training on the GPU -- are reported beside it
+under "forms".
 
   python bench.py [--gpus 1] [--steps 48] [--warmup 4] [--no-forms] [--no-cpu]
 
@@ -23,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -34,7 +38,20 @@ ALGO_BYTES = {"fp64": 1016, "mixed": 756}
 CPU_SAMPLE_TILES = 16384
 
 
-def run_form(n, steps, warmup, precision, log, class_order=True):
+HEADLINE = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes", "canopy%fgpp",
+            "ssnow%runoff", "rad%trad"]
+
+
+def flux_errors(got, ref):
+    """max, 99.9th and 99th percentile of |got - ref| per headline flux."""
+    out = {}
+    for k in HEADLINE:
+        d = np.abs(np.asarray(got[k], dtype=np.float64) - np.asarray(ref[k], dtype=np.float64)).ravel()
+        out[k] = {"max": float(d.max()), "p99.9": float(np.quantile(d, 0.999)), "p99": float(np.quantile(d, 0.99))}
+    return out
+
+
+def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None, keep=False):
     from cable_amd import gpu
     from cable_amd.workload import Workload
     nsteps = max(steps, warmup, 1)
@@ -43,26 +60,39 @@ def run_form(n, steps, warmup, precision, log, class_order=True):
     log(f"[bench] workload {n} tiles x {nsteps} steps generated in {time.perf_counter() - t:.1f}s")
     prec = gpu.PREC_FP64 if precision == "fp64" else gpu.PREC_MIXED
     order = gpu.class_order(wl.iveg, wl.isoilm) if class_order else None
+
+    def go(g, k):
+        if hybrid is None:
+            g.run(0, k, wl.dels)
+        else:
+            g.set_surrogate(hybrid["params"], hybrid["norm"])
+            g.run_hybrid(0, k, wl.dels, hybrid["max_passes"], hybrid["resid_max"])
     if warmup:
         g = gpu.from_workload(wl, precision=prec, full=False, order=order)
-        g.run(0, warmup, wl.dels)
+        go(g, warmup)
         g.sync()
         g.close()
     g = gpu.from_workload(wl, precision=prec, full=False, order=order)
     g.sync()
     t0 = time.perf_counter()
-    g.run(0, steps, wl.dels)
+    go(g, steps)
     g.sync()
     wall = time.perf_counter() - t0
     ev_ms = g.last_run_ms()
     passes = g.dryleaf_passes()
+    extra = {}
+    if hybrid is not None:
+        fb, ts = g.hybrid_counters()
+        extra["fallback_fraction"] = fb / ts
+    if keep:
+        extra["_out"] = {k: g.download(k) for k in HEADLINE}
     g.close()
     per_launch_s = ev_ms * 1e-3 / steps
     achieved = ALGO_BYTES[precision] * n / per_launch_s / 1e9
     log(f"[bench] {precision} n={n}: wall {wall * 1e3:.2f} ms for {steps} steps, "
         f"events {ev_ms:.2f} ms, {n * steps / wall:.4g} tile-steps/s")
-    return {"tiles": n, "steps": steps, "value": n * steps / wall, "ms_per_step": wall * 1e3 / steps,
-            "kernel_ms": ev_ms / steps, "achieved_GBs": achieved, "dryleaf_passes": passes}
+    return dict({"tiles": n, "steps": steps, "value": n * steps / wall, "ms_per_step": wall * 1e3 / steps,
+                 "kernel_ms": ev_ms / steps, "achieved_GBs": achieved, "dryleaf_passes": passes}, **extra)
 
 
 def profiled_traffic(tiles, precision):
@@ -142,9 +172,26 @@ def main():
     head = run_form(args.tiles, args.steps, args.warmup, "fp64", log, co)
     forms = {f"fp64_{args.tiles}": head}
     if not args.no_forms:
-        forms["fp64_1048576"] = run_form(1048576, args.steps, args.warmup, "fp64", log, co)
-        forms["mixed_1048576"] = run_form(1048576, args.steps, args.warmup, "mixed", log, co)
-        forms["mixed_vs_fp64_at_1048576"] = forms["mixed_1048576"]["value"] / forms["fp64_1048576"]["value"]
+        from cable_amd import surrogate
+        f64 = run_form(1048576, args.steps, args.warmup, "fp64", log, co, keep=True)
+        mix = run_form(1048576, args.steps, args.warmup, "mixed", log, co, keep=True)
+        # the surrogate: samples from the plain mixed kernel on a fresh workload
+        # of the same distribution, trained by the GPU kernels
+        params, norm, info = surrogate.train_on_gpu(n_tiles=262144, nsteps=64, epochs=2.0, log=log)
+        flops = 3 * 2 * (24 * 128 + 2 * 128 * 128 + 128 * 8) * info["batches"] * 2048
+        info["training_TFLOPs"] = flops / info["train_s"] / 1e12
+        hyb = run_form(1048576, args.steps, args.warmup, "mixed", log, co, keep=True,
+                       hybrid={"params": params, "norm": norm, "max_passes": 4, "resid_max": 0.5})
+        ref = f64.pop("_out")
+        mix["flux_error_vs_fp64_after_run"] = flux_errors(mix.pop("_out"), ref)
+        hyb["flux_error_vs_fp64_after_run"] = flux_errors(hyb.pop("_out"), ref)
+        hyb["surrogate_training"] = info
+        hyb["max_passes"], hyb["resid_max_Wm2"] = 4, 0.5
+        forms["fp64_1048576"] = f64
+        forms["mixed_1048576"] = mix
+        forms["hybrid_1048576"] = hyb
+        forms["mixed_vs_fp64_at_1048576"] = mix["value"] / f64["value"]
+        forms["hybrid_vs_fp64_at_1048576"] = hyb["value"] / f64["value"]
     traffic, traffic_src = profiled_traffic(args.tiles, "fp64") if not args.caller_order else (None, None)
     out = {
         "metric": METRIC,

@@ -1,0 +1,88 @@
+"""The surrogate's use in the step (mixed-precision contexts, on the GPU):
+sample capture, the hybrid step's fallback machinery, and what the trained
+network gains over the same capped iteration without it."""
+import numpy as np
+import pytest
+
+from cable_amd.workload import Workload
+
+pytestmark = pytest.mark.gpu
+
+HEADLINE = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes", "canopy%fgpp",
+            "ssnow%runoff", "rad%trad"]
+
+
+def _ctx(wl, precision=1):
+    from cable_amd import gpu
+    return gpu.from_workload(wl, precision=precision, full=False, order=gpu.class_order(wl.iveg, wl.isoilm))
+
+
+def _outputs(g, names=HEADLINE + ["ssnow%tgg", "ssnow%wb"]):
+    return {k: g.download(k) for k in names}
+
+
+def test_capture_keeps_the_plain_trajectory_and_writes_samples():
+    print("[hybrid] capture", flush=True)
+    wl = Workload(8192, 16)
+    g = _ctx(wl)
+    n = g.run_capture(0, 16, wl.dels, 8192 * 16)
+    a = _outputs(g)
+    x, y = g.copy_samples(0, n)
+    g.close()
+    g = _ctx(wl)
+    g.run(0, 16, wl.dels)
+    b = _outputs(g)
+    g.close()
+    # at most one sample per tile-step of a type that can carry leaves
+    # (iveg < 14), and only where the canopy above the snow has leaves
+    veg = (wl.iveg < 14).sum()
+    assert 0.5 * veg * 16 <= n <= veg * 16
+    assert np.isfinite(x).all() and np.isfinite(y).all()
+    # positive leaf-surface CO2; leaf temperature within 20 K of the air
+    assert (y[:, 1] > 0).all() and (y[:, 2] > 0).all()
+    assert np.abs(y[:, 0]).max() < 20.0
+    for k in a:
+        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-2, err_msg=k)
+
+
+def test_uncapped_hybrid_without_network_is_the_plain_step():
+    """max_passes = 20 and no network: the hybrid step is the plain method,
+    nothing falls back."""
+    print("[hybrid] uncapped without network", flush=True)
+    wl = Workload(8192, 12)
+    g = _ctx(wl)
+    g.run_hybrid(0, 12, wl.dels, max_passes=20, resid_max=0.5, use_surrogate=False)
+    fb, ts = g.hybrid_counters()
+    a = _outputs(g)
+    g.close()
+    g = _ctx(wl)
+    g.run(0, 12, wl.dels)
+    b = _outputs(g)
+    g.close()
+    assert fb == 0 and ts == 8192 * 12
+    for k in a:
+        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-2, err_msg=k)
+
+
+def test_network_warm_start_cuts_fallbacks_and_passes():
+    """Train on samples of one workload, run another: at the same pass cap the
+    network's starting point leaves fewer tiles to the fallback, and with no
+    cap it needs fewer leaf passes, than the reference's own start."""
+    from cable_amd import surrogate as S
+    print("[hybrid] network gain", flush=True)
+    params, norm, info = S.train_on_gpu(n_tiles=65536, nsteps=48, epochs=2.0,
+                                        log=lambda m: print(m, flush=True))
+    assert info["loss_last"] < 0.2 * info["loss_first"]
+    wl = Workload(65536, 24)
+    res = {}
+    for use in (True, False):
+        for cap in (3, 20):
+            g = _ctx(wl)
+            g.set_surrogate(params, norm)
+            g.run_hybrid(0, 24, wl.dels, max_passes=cap, resid_max=0.5, use_surrogate=use)
+            fb, ts = g.hybrid_counters()
+            res[(use, cap)] = (fb / ts, g.dryleaf_passes())
+            g.close()
+    print({f"surrogate={u} cap={c}": (round(v[0], 4), v[1]) for (u, c), v in res.items()}, flush=True)
+    assert res[(True, 3)][0] < 0.5 * res[(False, 3)][0]
+    assert res[(True, 20)][1] < res[(False, 20)][1]
