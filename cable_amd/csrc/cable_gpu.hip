@@ -532,7 +532,11 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
   if (use_surrogate && !c->swt) return fail(CABLE_E_STATE, "cable_gpu_run_hybrid: no surrogate set");
   if (max_passes < 1 || max_passes > 20) return fail(CABLE_E_ARG, "cable_gpu_run_hybrid: max_passes %d", max_passes);
   if ((r = ensure_hyb(c))) return r;
-  if (!c->pred) HIPCHECK(hipMalloc(&c->pred, sizeof(float) * cbm_mlp::NOUT * (size_t)c->mp));
+  if (!c->pred) {
+    // zero-filled: the inference kernel skips workgroups without leaves
+    HIPCHECK(hipMalloc(&c->pred, sizeof(float) * cbm_mlp::NOUT * (size_t)c->mp));
+    HIPCHECK(hipMemset(c->pred, 0, sizeof(float) * cbm_mlp::NOUT * (size_t)c->mp));
+  }
   if (!c->queue) HIPCHECK(hipMalloc(&c->queue, sizeof(int) * (size_t)c->mp));
   if (!c->qlen) HIPCHECK(hipMalloc(&c->qlen, sizeof(unsigned)));
   if (!c->qtotal) HIPCHECK(hipMalloc(&c->qtotal, sizeof(unsigned long long)));

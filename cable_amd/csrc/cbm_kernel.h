@@ -328,6 +328,14 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = lane & 31, kq = 4 * (lane >> 5);
   const int64_t base = ((int64_t)blockIdx.x * NWAVES + wave) * 32;
   const float *norm = a.norm;
+  // a workgroup whose 128 tiles carry no leaves has nothing to predict (the
+  // warm start applies to vegetated tiles only); the decision is uniform
+  // over the workgroup, so it leaves before any barrier
+  {
+    const int64_t t = (int64_t)blockIdx.x * NWAVES * 32 + threadIdx.x;
+    const bool leafy = t < a.mp && threadIdx.x < NWAVES * 32 && ld<int>(a.F[CABLE_F_veg_iveg], t) < 14;
+    if (!__syncthreads_or(leafy)) return;
+  }
   if (lane < 32) {
     const int64_t t = base + lane;
     float x[NIN];
@@ -352,20 +360,24 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
     const float *bias = params + LAYER_B[l];
     v16f acc[4];
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) {
+    for (int nb = 0; nb < 4; nb++)
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[nb][r] = 0.0f;
-      for (int k0 = 0; k0 < K; k0 += 8) {
-        v4s av, bv;
-        const uint16_t *src = (l == 0) ? &xs[wave][row][k0 + kq] : &hs[wave][row][k0 + kq];
+    // K outer: one A operand from LDS feeds the four 32-column blocks, whose
+    // weight operands are loaded together and whose products are independent
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      v4s av, bv[4];
+      const uint16_t *src = (l == 0) ? &xs[wave][row][k0 + kq] : &hs[wave][row][k0 + kq];
+#pragma unroll
+      for (int e = 0; e < 4; e++) av[e] = (short)src[e];
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) {
         const uint16_t *wsrc = W + (int64_t)(nb * 32 + row) * K + k0 + kq;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          av[e] = (short)src[e];
-          bv[e] = (short)wsrc[e];
-        }
-        acc[nb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc[nb], 0, 0, 0);
+        for (int e = 0; e < 4; e++) bv[nb][e] = (short)wsrc[e];
       }
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) acc[nb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv[nb], acc[nb], 0, 0, 0);
     }
     __syncthreads();  // every wave has read this layer's input
 #pragma unroll

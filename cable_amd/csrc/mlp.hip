@@ -47,26 +47,31 @@ int fail(int code, const char *fmt, ...) {
 
 enum Epi { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_SILU_GRAD = 3 };
 
-// C[M x N] = A[M x K] . B[K x N]; one wave per 32x32 block of C
+// C[M x N] = A[M x K] . B[K x N]; one wave per 32x32 block of C.  With
+// kchunk < K the product is split along K: block z sums k in
+// [z*kchunk, (z+1)*kchunk) and stores its partial C at C + z*M*N (EPI_STORE
+// only); reduce_kernel adds the partials in order.
 template <class TA, class TB>
 __global__ void __launch_bounds__(64) gemm_kernel(const TA *__restrict__ A, int64_t sam, int64_t sak,
                                                   const TB *__restrict__ B, int64_t sbk, int64_t sbn,
                                                   float *__restrict__ C, float *__restrict__ H, int M,
                                                   int N, int K, const float *__restrict__ bias,
-                                                  const float *__restrict__ Zsaved, int epi) {
+                                                  const float *__restrict__ Zsaved, int epi, int kchunk) {
   const int lane = threadIdx.x;
   const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
   const int ia = i0 + (lane & 31), jb = j0 + (lane & 31), kq = 4 * (lane >> 5);
+  const int kbeg = blockIdx.z * kchunk, kend = kbeg + kchunk < K ? kbeg + kchunk : K;
+  C += (int64_t)blockIdx.z * M * N;
   v16f acc;
 #pragma unroll
   for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-  for (int k0 = 0; k0 < K; k0 += 8) {
+  for (int k0 = kbeg; k0 < kend; k0 += 8) {
     v4s a, b;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int k = k0 + kq + e;
-      const float av = (ia < M && k < K) ? to_f32(A[(int64_t)ia * sam + (int64_t)k * sak]) : 0.0f;
-      const float bv = (jb < N && k < K) ? to_f32(B[(int64_t)k * sbk + (int64_t)jb * sbn]) : 0.0f;
+      const float av = (ia < M && k < kend) ? to_f32(A[(int64_t)ia * sam + (int64_t)k * sak]) : 0.0f;
+      const float bv = (jb < N && k < kend) ? to_f32(B[(int64_t)k * sbk + (int64_t)jb * sbn]) : 0.0f;
       a[e] = (short)f32_to_bf16(av);
       b[e] = (short)f32_to_bf16(bv);
     }
@@ -94,13 +99,31 @@ __global__ void __launch_bounds__(64) gemm_kernel(const TA *__restrict__ A, int6
   }
 }
 
-// db[j] = sum_i dZ[i][j]: one thread per column, FP32 in row order
-__global__ void colsum_kernel(const float *__restrict__ dZ, int M, int N, float *__restrict__ db) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
+// out[i] = sum_z part[z][i] for i < n, z < nz (split-K partials, in order)
+__global__ void reduce_kernel(const float *__restrict__ part, int nz, int64_t n, float *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   float s = 0.0f;
-  for (int i = 0; i < M; i++) s += dZ[(int64_t)i * N + j];
-  db[j] = s;
+  for (int z = 0; z < nz; z++) s += part[z * n + i];
+  out[i] = s;
+}
+
+// db[j] = sum_i dZ[i][j]: 32 columns per workgroup of 256 threads; eight
+// row groups summed in FP32, then combined through LDS
+__global__ void __launch_bounds__(256) colsum_kernel(const float *__restrict__ dZ, int M, int N,
+                                                     float *__restrict__ db) {
+  __shared__ float part[8][32];
+  const int c = threadIdx.x & 31, r = threadIdx.x >> 5, j = blockIdx.x * 32 + c;
+  float s = 0.0f;
+  if (j < N)
+    for (int i = r; i < M; i += 8) s += dZ[(int64_t)i * N + j];
+  part[r][c] = s;
+  __syncthreads();
+  if (r == 0 && j < N) {
+    float t = 0.0f;
+    for (int q = 0; q < 8; q++) t += part[q][c];
+    db[j] = t;
+  }
 }
 
 // normalised batch: x = (raw[idx] - mu) / sd, y likewise; rows past n are 0
@@ -167,6 +190,7 @@ struct cable_mlp {
   // activations of one batch
   float *x = nullptr, *y = nullptr, *z[3] = {nullptr, nullptr, nullptr}, *h[3] = {nullptr, nullptr, nullptr};
   float *out = nullptr, *dy = nullptr, *dh = nullptr, *dz = nullptr, *loss = nullptr;
+  float *part = nullptr;      // split-K partials of the weight gradients
   // training samples (device)
   float *xs = nullptr, *ys = nullptr;
   int64_t nsamples = 0;
@@ -182,12 +206,17 @@ namespace {
 
 template <class TA, class TB>
 hipError_t gemm(hipStream_t s, const TA *A, int64_t sam, int64_t sak, const TB *B, int64_t sbk, int64_t sbn,
-                float *C, float *H, int M, int N, int K, const float *bias, const float *Z, int epi) {
-  dim3 grid((M + 31) / 32, (N + 31) / 32);
+                float *C, float *H, int M, int N, int K, const float *bias, const float *Z, int epi,
+                int nsplit = 1) {
+  const int kchunk = ((K + nsplit - 1) / nsplit + 7) / 8 * 8;
+  dim3 grid((M + 31) / 32, (N + 31) / 32, nsplit);
   hipLaunchKernelGGL((gemm_kernel<TA, TB>), grid, dim3(64), 0, s, A, sam, sak, B, sbk, sbn, C, H, M, N, K, bias,
-                     Z, epi);
+                     Z, epi, kchunk);
   return hipGetLastError();
 }
+
+// weight gradients: K (the batch) split in chunks of KSPLIT_ROWS rows
+constexpr int KSPLIT_ROWS = 256;
 
 // forward pass of n rows of m->x into m->out (pre-activations kept)
 int forward(cable_mlp *m, int n) {
@@ -221,10 +250,14 @@ int backward(cable_mlp *m, int n, float *loss) {
   for (int l = 3; l >= 0; l--) {
     const int K = LAYER_IN[l], N = LAYER_OUT[l];
     const float *hin = l == 0 ? m->x : m->h[l - 1];
-    // dW[l] (K x N) = hin^T (K x n) . dcur (n x N)
-    HIPCHECK(gemm(m->stream, hin, (int64_t)1, (int64_t)K, dcur, (int64_t)N, (int64_t)1, m->g + LAYER_W[l],
-                  (float *)nullptr, K, N, n, (const float *)nullptr, (const float *)nullptr, EPI_STORE));
-    hipLaunchKernelGGL(colsum_kernel, dim3((N + 127) / 128), dim3(128), 0, m->stream, dcur, n, N,
+    // dW[l] (K x N) = hin^T (K x n) . dcur (n x N), split along the batch
+    const int nsplit = (n + KSPLIT_ROWS - 1) / KSPLIT_ROWS;
+    HIPCHECK(gemm(m->stream, hin, (int64_t)1, (int64_t)K, dcur, (int64_t)N, (int64_t)1, m->part,
+                  (float *)nullptr, K, N, n, (const float *)nullptr, (const float *)nullptr, EPI_STORE, nsplit));
+    hipLaunchKernelGGL(reduce_kernel, dim3((K * N + 255) / 256), dim3(256), 0, m->stream, m->part, nsplit,
+                       (int64_t)K * N, m->g + LAYER_W[l]);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(colsum_kernel, dim3((N + 31) / 32), dim3(256), 0, m->stream, dcur, n, N,
                        m->g + LAYER_B[l]);
     HIPCHECK(hipGetLastError());
     if (l == 0) break;
@@ -287,6 +320,7 @@ int cable_mlp_create(cable_mlp **out, int batch) {
   al((void **)&m->dh, sizeof(float) * (size_t)batch * HID);
   al((void **)&m->dz, sizeof(float) * (size_t)batch * HID);
   al((void **)&m->loss, sizeof(float));
+  al((void **)&m->part, sizeof(float) * HID * HID * (size_t)((batch + KSPLIT_ROWS - 1) / KSPLIT_ROWS));
   if (e != hipSuccess) {
     cable_mlp_destroy(m);
     return fail(CABLE_MLP_E_HIP, "cable_mlp_create: %s", hipGetErrorString(e));
@@ -303,7 +337,7 @@ int cable_mlp_create(cable_mlp **out, int batch) {
 int cable_mlp_destroy(cable_mlp *m) {
   if (!m) return CABLE_MLP_OK;
   void *ps[] = {m->w, m->g, m->m, m->v, m->wb, m->norm, m->x, m->y, m->z[0], m->z[1], m->z[2], m->h[0], m->h[1],
-                m->h[2], m->out, m->dy, m->dh, m->dz, m->loss, m->idx, m->losses};
+                m->h[2], m->out, m->dy, m->dh, m->dz, m->loss, m->idx, m->losses, m->part};
   for (void *p : ps)
     if (p) hipFree(p);
   if (m->own_samples) {
