@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--tiles", type=int, default=262144)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="path of the headline line (fp64: the plain FP64 step)")
     ap.add_argument("--no-forms", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--caller-order", action="store_true",
@@ -169,8 +171,9 @@ def main():
     # the CPU leg first, while this process has not touched the GPU
     cpu = None if args.no_cpu else cpu_baseline(args.steps, log)
     co = not args.caller_order
-    head = run_form(args.tiles, args.steps, args.warmup, "fp64", log, co)
-    forms = {f"fp64_{args.tiles}": head}
+    prec = args.precision
+    head = run_form(args.tiles, args.steps, args.warmup, prec, log, co)
+    forms = {f"{prec}_{args.tiles}": head}
     if not args.no_forms:
         from cable_amd import surrogate
         f64 = run_form(1048576, args.steps, args.warmup, "fp64", log, co, keep=True)
@@ -192,7 +195,8 @@ def main():
         forms["hybrid_1048576"] = hyb
         forms["mixed_vs_fp64_at_1048576"] = mix["value"] / f64["value"]
         forms["hybrid_vs_fp64_at_1048576"] = hyb["value"] / f64["value"]
-    traffic, traffic_src = profiled_traffic(args.tiles, "fp64") if not args.caller_order else (None, None)
+    traffic, traffic_src = profiled_traffic(args.tiles, prec) if not args.caller_order else (None, None)
+    what = {"fp64": "cbm plain FP64", "mixed": "cbm mixed precision (FP32 canopy, FP64 solves and balances)"}[prec]
     out = {
         "metric": METRIC,
         "value": head["value"],
@@ -204,10 +208,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if prec == "fp64" else "f32+f64",
         "data": "synthetic (seeded forcing, parameters from the reference's PFT and soil tables)",
-        "config": {"workload": f"cbm plain FP64, {args.tiles} tiles x {args.steps} half-hourly steps",
-                   "tiles": args.tiles, "precision": "fp64", "path": "device-resident run",
+        "config": {"workload": f"{what}, {args.tiles} tiles x {args.steps} half-hourly steps",
+                   "tiles": args.tiles, "precision": prec, "path": "device-resident run",
                    "tile_order": "caller" if args.caller_order else "grouped by vegetation and soil type"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,

@@ -216,3 +216,26 @@ def train_on_gpu(n_tiles=262144, nsteps=64, epochs=2.0, batch=2048, lr=2e-3, see
     net.close()
     g.close()
     return params, norm, info
+
+
+def main(argv=None):
+    """python -m cable_amd.surrogate [--tiles N] [--steps K] [--epochs E] [--out F.npz]:
+    capture samples and train on the GPU; prints the training summary as JSON."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--tiles", type=int, default=262144)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--epochs", type=float, default=2.0)
+    ap.add_argument("--out", default=None, help="save params and norm (numpy .npz)")
+    args = ap.parse_args(argv)
+    params, norm, info = train_on_gpu(args.tiles, args.steps, args.epochs,
+                                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    if args.out:
+        np.savez(args.out, params=params, norm=norm)
+    print(json.dumps(info), flush=True)
+
+
+if __name__ == "__main__":
+    main()
