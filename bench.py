@@ -34,7 +34,7 @@ METRIC = "tile-steps/sec on 1 MI355X: hybrid vs plain FP64, at ≤1 W/m² flux e
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per tile-step: prognostic state read and written once,
 # 8 forcing fields and the per-tile parameters read once, 8 outputs written
-ALGO_BYTES = {"fp64": 1016, "mixed": 756}
+ALGO_BYTES = {"fp64": 1016, "mixed": 756, "hybrid": 764}
 CPU_SAMPLE_TILES = 16384
 
 
@@ -133,6 +133,84 @@ def cpu_baseline(steps, log):
                       f"one process; {sec:.2f} s"}
 
 
+TRAIN = {"n_tiles": 262144, "nsteps": 96, "epochs": 2.0, "max_samples": 16777216}
+
+
+def production(args, log):
+    """The hybrid production run: 4,194,304 tiles x 336 half-hourly steps
+    (one week) with forcing generated on the device in 48-step windows, run
+    side by side in plain FP64, mixed precision and the surrogate-warm-started
+    hybrid; returns the JSON line (value: the hybrid's tile-steps/s)."""
+    from cable_amd import gpu, surrogate
+    from cable_amd.workload import DEFAULT_SEED, Workload
+    n, total, win = args.tiles, args.steps, 48
+    t = time.perf_counter()
+    wl = Workload(n, 1, forcing=False)
+    log(f"[bench] production workload {n} tiles generated in {time.perf_counter() - t:.1f}s")
+    params, norm, info = surrogate.train_on_gpu(log=log, **TRAIN)
+    order = gpu.class_order(wl.iveg, wl.isoilm)
+    ctx = {"fp64": gpu.from_workload(wl, precision=gpu.PREC_FP64, full=False, order=order),
+           "mixed": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order),
+           "hybrid": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order)}
+    ctx["hybrid"].set_surrogate(params, norm)
+    wall = {k: 0.0 for k in ctx}
+    dev_ms = {k: 0.0 for k in ctx}
+    passes = {k: 0 for k in ctx}
+    fb = ts = 0
+    synth_s = 0.0
+    for k0 in range(0, total, win):
+        nk = min(win, total - k0)
+        for name, g in ctx.items():
+            t = time.perf_counter()
+            g.synth_forcing(k0, nk, DEFAULT_SEED, wl.t0, wl.lat, dels=wl.dels)
+            synth_s += time.perf_counter() - t
+            g.sync()
+            t = time.perf_counter()
+            if name == "hybrid":
+                g.run_hybrid(k0, nk, wl.dels, 4, 0.5)
+            else:
+                g.run(k0, nk, wl.dels)
+            g.sync()
+            wall[name] += time.perf_counter() - t
+            dev_ms[name] += g.last_run_ms()
+            passes[name] += g.dryleaf_passes()
+            if name == "hybrid":
+                f, s_ = g.hybrid_counters()
+                fb, ts = fb + f, ts + s_
+        log(f"[bench] production steps {k0}..{k0 + nk - 1} done")
+    out = {k: {h: g.download(h) for h in HEADLINE} for k, g in ctx.items()}
+    for g in ctx.values():
+        g.close()
+    forms = {}
+    for k in ctx:
+        forms[f"{k}_{n}"] = {"tiles": n, "steps": total, "value": n * total / wall[k],
+                             "ms_per_step": wall[k] * 1e3 / total, "kernel_ms": dev_ms[k] / total,
+                             "dryleaf_passes": passes[k]}
+    forms[f"mixed_{n}"]["flux_error_vs_fp64_after_run"] = flux_errors(out["mixed"], out["fp64"])
+    forms[f"hybrid_{n}"]["flux_error_vs_fp64_after_run"] = flux_errors(out["hybrid"], out["fp64"])
+    forms[f"hybrid_{n}"]["fallback_fraction"] = fb / ts
+    forms[f"hybrid_{n}"]["max_passes"], forms[f"hybrid_{n}"]["resid_max_Wm2"] = 4, 0.5
+    forms[f"hybrid_{n}"]["surrogate_training"] = info
+    forms["hybrid_vs_fp64"] = wall["fp64"] / wall["hybrid"]
+    forms["mixed_vs_fp64"] = wall["fp64"] / wall["mixed"]
+    forms["forcing_synthesis_s"] = synth_s
+    hyb = forms[f"hybrid_{n}"]
+    achieved = ALGO_BYTES["hybrid"] * n / (hyb["kernel_ms"] * 1e-3) / 1e9
+    return {
+        "metric": METRIC, "value": hyb["value"], "unit": "tile-steps/s", "n_gpus": 1, "steps": total,
+        "warmup": 0, "ms_per_step": hyb["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16+f32+f64",
+        "data": "synthetic (forcing generated on the device, parameters from the reference's PFT and soil tables)",
+        "config": {"workload": f"hybrid production run, {n} tiles x {total} half-hourly steps",
+                   "tiles": n, "precision": "mixed + BF16 surrogate warm start", "path": "device-resident run",
+                   "tile_order": "grouped by vegetation and soil type", "forcing_window_steps": win},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+        "forms": forms,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +223,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--caller-order", action="store_true",
                     help="keep the tiles in the caller's order on the device (no class grouping)")
+    ap.add_argument("--production", action="store_true",
+                    help="the hybrid production run instead: 4,194,304 tiles x 336 steps by default "
+                         "(--tiles, --steps), FP64, mixed and hybrid side by side")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes (torch), "
                          "the reference kernel for scaling HBM counters, and exit")
@@ -168,6 +249,13 @@ def main():
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    if args.production:
+        if args.tiles == 262144:
+            args.tiles = 4194304
+        if args.steps == 48:
+            args.steps = 336
+        print(json.dumps(production(args, log)), flush=True)
+        return
     # the CPU leg first, while this process has not touched the GPU
     cpu = None if args.no_cpu else cpu_baseline(args.steps, log)
     co = not args.caller_order
@@ -180,7 +268,7 @@ def main():
         mix = run_form(1048576, args.steps, args.warmup, "mixed", log, co, keep=True)
         # the surrogate: samples from the plain mixed kernel on a fresh workload
         # of the same distribution, trained by the GPU kernels
-        params, norm, info = surrogate.train_on_gpu(n_tiles=262144, nsteps=64, epochs=2.0, log=log)
+        params, norm, info = surrogate.train_on_gpu(log=log, **TRAIN)
         flops = 3 * 2 * (24 * 128 + 2 * 128 * 128 + 128 * 8) * info["batches"] * 2048
         info["training_TFLOPs"] = flops / info["train_s"] / 1e12
         hyb = run_form(1048576, args.steps, args.warmup, "mixed", log, co, keep=True,

@@ -81,6 +81,7 @@ struct cable_gpu_ctx {
   void **d_tab_run = nullptr;       // `dev` with forcing ids -> resident forcing
   std::vector<void *> forc;         // resident forcing arrays (8)
   int nsteps_forc = 0;
+  int forc_k0 = 0;                  // step of the first resident forcing slot
   double ca = 400e-6, doy = 1.0;
   double zse[6] = {0}, zshh[7] = {0}, ratecp[3] = {0}, ratecs[2] = {0};
   double *d_vtab = nullptr, *d_stab = nullptr;
@@ -394,8 +395,185 @@ int cable_gpu_set_forcing(cable_gpu_ctx *c, int nsteps, const void *fsd1, const 
   HIPCHECK(hipMemcpyAsync(c->d_tab_run, tab.data(), sizeof(void *) * CABLE_NFIELDS, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   c->nsteps_forc = nsteps;
+  c->forc_k0 = 0;
   c->ca = ca;
   c->doy = doy;
+  return CABLE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// synthetic forcing generated on the device
+
+}  // extern "C"
+
+namespace {
+// SplitMix64's increment and finaliser (Steele, Lea and Flood, "Fast
+// splittable pseudorandom number generators", OOPSLA 2014): mix64(0) is the
+// first output of SplitMix64 seeded with 0, 0xE220A8397B1DCDAF
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+constexpr int NSYNTH = 8;  // draws per tile-step: sw, lw, rh, ua (2), rain occurrence, amount, psurf
+struct SynthKeys {
+  uint64_t k[NSYNTH];
+};
+
+// U(0,1), open at both ends, of draw `s` for (caller tile, step)
+__device__ inline double synth_u(const SynthKeys &keys, int s, int64_t tile, int64_t step) {
+  const uint64_t h = mix64(keys.k[s] ^ ((uint64_t)step << 32) ^ (uint64_t)tile);
+  return ((double)(h >> 11) + 0.5) * 0x1.0p-53;
+}
+
+// saturation vapour pressure (hPa) and specific humidity from relative
+// humidity, src/offline/cable_checks.F90:633-696
+__device__ inline double synth_svp(double tk) {
+  const double l10 = log(10.0);
+  if (tk - 273.15 < -20.0) {
+    const double toot = 273.15 / tk, toto = 1.0 / toot;
+    const double eilog = -9.09718 * (toot - 1) - 3.56654 * (log(toot) / l10) + 0.876793 * (1 - toto) +
+                         (log(6.1071) / l10);
+    return pow(10.0, eilog);
+  }
+  const double tsot = 373.15 / tk;
+  double ew = -7.90298 * (tsot - 1) + 5.02808 * (log(tsot) / l10);
+  ew = ew - 1.3816e-07 * (pow(10.0, 11.344 * (1 - (1 / tsot))) - 1);
+  ew = ew + 0.0081328 * (pow(10.0, -3.49149 * (tsot - 1)) - 1);
+  ew = ew + (log(1013.246) / l10);
+  return pow(10.0, ew);
+}
+
+// one step of the synthetic met of every tile, in the forcing reader's
+// variables: SWdown from the diurnal cycle, split evenly between the two
+// bands (src/offline/cable_input.F90:1880-1883); Tair from the tile's mean
+// and a +-5 K diurnal swing; qv from RH ~ U(30, 90)% by rh_sh; lognormal wind
+// (mean 3 m/s, sigma 0.5, Box-Muller); Bernoulli(0.1) x Exp(mean 2 mm/h)
+// precipitation in mm per step; PSurf ~ U(950, 1020) hPa; LWdown ~
+// U(250, 420) W/m2; coszen by sinbet (src/science/radiation/cbl_sinbet.F90:8-28)
+template <class R>
+__global__ void __launch_bounds__(256)
+    synth_kernel(int64_t mp, int k_first, SynthKeys keys, double dels, double doy, double hod0,
+                 const int64_t *__restrict__ tix, const double *__restrict__ t0, const double *__restrict__ lat,
+                 R *fsd1, R *fld, R *tk, R *qv, R *ua, R *pmb, R *precip, R *coszen) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= mp) return;
+  const int64_t k = k_first + (int64_t)blockIdx.y, o = (int64_t)blockIdx.y * mp + j, t = tix[j];
+  const double two_pi = 2.0 * M_PI;
+  const double hod = hod0 + (double)k * (dels / 3600.0);
+  const double diurnal = fmax(0.0, 1000.0 * sin(two_pi * (hod - 6.0) / 24.0));
+  const double sw = diurnal * (0.5 + 0.5 * synth_u(keys, 0, t, k));
+  const double tair = t0[j] + 5.0 * sin(two_pi * (hod - 9.0) / 24.0);
+  const double ps = 950.0 + 70.0 * synth_u(keys, 7, t, k);
+  const double es = synth_svp(tair);
+  const double rh = 100.0 * (0.3 + 0.6 * synth_u(keys, 2, t, k));
+  const double q = (rh / 100.0) * (0.622 * es / (ps - es));
+  const double z = sqrt(-2.0 * log(synth_u(keys, 3, t, k))) * cos(two_pi * synth_u(keys, 4, t, k));
+  const double sigma = 0.5;
+  const double wind = exp((log(3.0) - 0.5 * sigma * sigma) + sigma * z);
+  const double rain = synth_u(keys, 5, t, k) < 0.1 ? 2.0 * -log(synth_u(keys, 6, t, k)) * (dels / 3600.0) : 0.0;
+  // sinbet with the reference's pi = 3.1415927 (src/params/cable_maths_constants_mod.F90:32)
+  const double pi_f = 3.1415927, d2r = pi_f / 180.0;
+  const double sindec = -sin(23.45 * d2r) * cos(2.0 * pi_f * (doy + 10.0) / 365.0);
+  const double cz = sin(d2r * lat[j]) * sindec +
+                    cos(d2r * lat[j]) * sqrt(1.0 - sindec * sindec) * cos(pi_f * (hod - 12.0) / 12.0);
+  fsd1[o] = (R)(0.5 * sw);
+  fld[o] = (R)(250.0 + 170.0 * synth_u(keys, 1, t, k));
+  tk[o] = (R)tair;
+  qv[o] = (R)q;
+  ua[o] = (R)wind;
+  pmb[o] = (R)ps;
+  precip[o] = (R)rain;
+  coszen[o] = (R)fmax(cz, 1e-8);
+}
+}  // namespace
+
+extern "C" {
+
+int cable_gpu_synth_forcing(cable_gpu_ctx *c, int k_first, int nsteps, uint64_t seed, double dels, double doy,
+                            double hod0, const double *t0, const double *lat) {
+  if (!c || !t0 || !lat || k_first < 0 || nsteps <= 0 || nsteps > 65535 || !(dels > 0))
+    return fail(CABLE_E_ARG, "cable_gpu_synth_forcing: bad arguments");
+  const size_t mp = (size_t)c->mp;
+  FieldMeta fm = {"forcing", 'F', 1, ROLE_FORC};
+  const size_t bytes = dev_elem(fm, c->precision) * (size_t)nsteps * mp;
+  if (c->nsteps_forc != nsteps || c->forc.size() != 8) {
+    for (void *p : c->forc)
+      if (p) HIPCHECK(hipFree(p));
+    c->forc.assign(8, nullptr);
+    c->nsteps_forc = 0;
+    for (int i = 0; i < 8; i++) HIPCHECK(hipMalloc(&c->forc[i], bytes));
+  }
+  // per-tile inputs in device order; the draws are keyed by the caller's tile
+  std::vector<int64_t> tix(mp);
+  std::vector<double> ht0(mp), hlat(mp);
+  for (size_t j = 0; j < mp; j++) {
+    tix[j] = c->order.empty() ? (int64_t)j : c->order[j];
+    ht0[j] = t0[tix[j]];
+    hlat[j] = lat[tix[j]];
+  }
+  int64_t *d_tix = nullptr;
+  double *d_t0 = nullptr, *d_lat = nullptr;
+  HIPCHECK(hipMalloc(&d_tix, sizeof(int64_t) * mp));
+  HIPCHECK(hipMalloc(&d_t0, sizeof(double) * mp));
+  HIPCHECK(hipMalloc(&d_lat, sizeof(double) * mp));
+  HIPCHECK(hipMemcpyAsync(d_tix, tix.data(), sizeof(int64_t) * mp, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_t0, ht0.data(), sizeof(double) * mp, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_lat, hlat.data(), sizeof(double) * mp, hipMemcpyHostToDevice, c->stream));
+  SynthKeys keys;
+  for (int s = 0; s < NSYNTH; s++) keys.k[s] = mix64(mix64(seed) ^ (uint64_t)(s + 1));
+  const dim3 grid((unsigned)((mp + 255) / 256), (unsigned)nsteps);
+  void **f = c->forc.data();
+  if (c->precision == CABLE_PREC_MIXED)
+    hipLaunchKernelGGL(synth_kernel<float>, grid, dim3(256), 0, c->stream, c->mp, k_first, keys, dels, doy, hod0,
+                       d_tix, d_t0, d_lat, (float *)f[0], (float *)f[1], (float *)f[2], (float *)f[3],
+                       (float *)f[4], (float *)f[5], (float *)f[6], (float *)f[7]);
+  else
+    hipLaunchKernelGGL(synth_kernel<double>, grid, dim3(256), 0, c->stream, c->mp, k_first, keys, dels, doy, hod0,
+                       d_tix, d_t0, d_lat, (double *)f[0], (double *)f[1], (double *)f[2], (double *)f[3],
+                       (double *)f[4], (double *)f[5], (double *)f[6], (double *)f[7]);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipFree(d_tix));
+  HIPCHECK(hipFree(d_t0));
+  HIPCHECK(hipFree(d_lat));
+  const int ids[8] = {CABLE_F_met_fsd, CABLE_F_met_fld, CABLE_F_met_tk, CABLE_F_met_qv,
+                      CABLE_F_met_ua, CABLE_F_met_pmb, CABLE_F_met_precip, CABLE_F_met_coszen};
+  std::vector<void *> tab = c->dev;
+  for (int i = 0; i < 8; i++) tab[ids[i]] = c->forc[i];
+  HIPCHECK(hipMemcpyAsync(c->d_tab_run, tab.data(), sizeof(void *) * CABLE_NFIELDS, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->nsteps_forc = nsteps;
+  c->forc_k0 = k_first;
+  c->ca = 400e-6;
+  c->doy = doy;
+  return CABLE_OK;
+}
+
+int cable_gpu_download_forcing(cable_gpu_ctx *c, int k, int which, void *host, int host_type) {
+  if (!c || !host || which < 0 || which >= 8) return fail(CABLE_E_ARG, "cable_gpu_download_forcing: bad arguments");
+  if (c->nsteps_forc == 0 || k < c->forc_k0 || k >= c->forc_k0 + c->nsteps_forc)
+    return fail(CABLE_E_ARG, "cable_gpu_download_forcing: step %d is not resident", k);
+  const size_t mp = (size_t)c->mp;
+  FieldMeta fm = {"forcing", 'F', 1, ROLE_FORC};
+  const size_t de = dev_elem(fm, c->precision);
+  std::vector<unsigned char> buf(de * mp);
+  HIPCHECK(hipMemcpyAsync(buf.data(), (unsigned char *)c->forc[which] + de * mp * (size_t)(k - c->forc_k0),
+                          buf.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  const int64_t *ord = c->order.empty() ? nullptr : c->order.data();
+  for (size_t j = 0; j < mp; j++) {
+    const size_t d = ord ? (size_t)ord[j] : j;
+    const double v = de == 8 ? ((double *)buf.data())[j] : (double)((float *)buf.data())[j];
+    if (host_type == CABLE_HOST_F64)
+      ((double *)host)[d] = v;
+    else if (host_type == CABLE_HOST_F32)
+      ((float *)host)[d] = (float)v;
+    else
+      return fail(CABLE_E_ARG, "cable_gpu_download_forcing: bad host type %d", host_type);
+  }
   return CABLE_OK;
 }
 
@@ -403,14 +581,14 @@ int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
   if (!c) return fail(CABLE_E_ARG, "cable_gpu_run: null context");
   if (!c->have_tables || c->nsteps_forc == 0)
     return fail(CABLE_E_STATE, "cable_gpu_run: class tables and forcing must be set first");
-  if (k0 < 0 || nsteps <= 0 || k0 + nsteps > c->nsteps_forc || !(dels > 0))
-    return fail(CABLE_E_ARG, "cable_gpu_run: steps %d..%d outside the %d resident forcing steps", k0,
-                k0 + nsteps - 1, c->nsteps_forc);
+  if (k0 < c->forc_k0 || nsteps <= 0 || k0 + nsteps > c->forc_k0 + c->nsteps_forc || !(dels > 0))
+    return fail(CABLE_E_ARG, "cable_gpu_run: steps %d..%d outside the resident forcing steps %d..%d", k0,
+                k0 + nsteps - 1, c->forc_k0, c->forc_k0 + c->nsteps_forc - 1);
   HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
   c->hyb_valid = false;   // the plain step does not keep the surrogate's inputs
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   for (int k = k0; k < k0 + nsteps; k++) {
-    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
+    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)(k - c->forc_k0) * c->mp);
     hipError_t e = launch(c, a, false);
     if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run launch: %s", hipGetErrorString(e));
     c->first_call = false;
@@ -435,9 +613,9 @@ int check_resident(cable_gpu_ctx *c, int k0, int nsteps, double dels, const char
   if (c->precision != CABLE_PREC_MIXED) return fail(CABLE_E_STATE, "%s: needs a mixed-precision context", who);
   if (!c->have_tables || c->nsteps_forc == 0)
     return fail(CABLE_E_STATE, "%s: class tables and forcing must be set first", who);
-  if (k0 < 0 || nsteps <= 0 || k0 + nsteps > c->nsteps_forc || !(dels > 0))
-    return fail(CABLE_E_ARG, "%s: steps %d..%d outside the %d resident forcing steps", who, k0, k0 + nsteps - 1,
-                c->nsteps_forc);
+  if (k0 < c->forc_k0 || nsteps <= 0 || k0 + nsteps > c->forc_k0 + c->nsteps_forc || !(dels > 0))
+    return fail(CABLE_E_ARG, "%s: steps %d..%d outside the resident forcing steps %d..%d", who, k0,
+                k0 + nsteps - 1, c->forc_k0, c->forc_k0 + c->nsteps_forc - 1);
   return CABLE_OK;
 }
 
@@ -468,7 +646,7 @@ int cable_gpu_run_capture(cable_gpu_ctx *c, int k0, int nsteps, double dels, int
   HIPCHECK(hipMemsetAsync(c->scount, 0, sizeof(unsigned long long), c->stream));
   HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
   for (int k = k0; k < k0 + nsteps; k++) {
-    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
+    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)(k - c->forc_k0) * c->mp);
     a.hyb = c->hyb;
     a.hyb_valid = c->hyb_valid ? 1 : 0;
     a.sx = c->sx;
@@ -544,7 +722,7 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
   HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   for (int k = k0; k < k0 + nsteps; k++) {
-    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)k * c->mp);
+    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)(k - c->forc_k0) * c->mp);
     a.hyb = c->hyb;
     a.hyb_valid = c->hyb_valid ? 1 : 0;
     a.norm = c->snorm;

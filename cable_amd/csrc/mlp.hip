@@ -108,21 +108,23 @@ __global__ void reduce_kernel(const float *__restrict__ part, int nz, int64_t n,
   out[i] = s;
 }
 
-// db[j] = sum_i dZ[i][j]: 32 columns per workgroup of 256 threads; eight
-// row groups summed in FP32, then combined through LDS
-__global__ void __launch_bounds__(256) colsum_kernel(const float *__restrict__ dZ, int M, int N,
-                                                     float *__restrict__ db) {
-  __shared__ float part[8][32];
+// column sums of dZ in row blocks: block (x, y) sums rows [y*rows, (y+1)*rows)
+// of 32 columns into part[y][j] (eight row groups combined through LDS);
+// reduce_kernel then adds the row blocks in order
+__global__ void __launch_bounds__(256) colsum_kernel(const float *__restrict__ dZ, int M, int N, int rows,
+                                                     float *__restrict__ part) {
+  __shared__ float acc[8][32];
   const int c = threadIdx.x & 31, r = threadIdx.x >> 5, j = blockIdx.x * 32 + c;
+  const int i0 = blockIdx.y * rows, i1 = i0 + rows < M ? i0 + rows : M;
   float s = 0.0f;
   if (j < N)
-    for (int i = r; i < M; i += 8) s += dZ[(int64_t)i * N + j];
-  part[r][c] = s;
+    for (int i = i0 + r; i < i1; i += 8) s += dZ[(int64_t)i * N + j];
+  acc[r][c] = s;
   __syncthreads();
   if (r == 0 && j < N) {
     float t = 0.0f;
-    for (int q = 0; q < 8; q++) t += part[q][c];
-    db[j] = t;
+    for (int q = 0; q < 8; q++) t += acc[q][c];
+    part[(int64_t)blockIdx.y * N + j] = t;
   }
 }
 
@@ -257,7 +259,10 @@ int backward(cable_mlp *m, int n, float *loss) {
     hipLaunchKernelGGL(reduce_kernel, dim3((K * N + 255) / 256), dim3(256), 0, m->stream, m->part, nsplit,
                        (int64_t)K * N, m->g + LAYER_W[l]);
     HIPCHECK(hipGetLastError());
-    hipLaunchKernelGGL(colsum_kernel, dim3((N + 31) / 32), dim3(256), 0, m->stream, dcur, n, N,
+    // db[l] = column sums of dcur; the partials reuse m->part (nsplit x N <= its size)
+    hipLaunchKernelGGL(colsum_kernel, dim3((N + 31) / 32, nsplit), dim3(256), 0, m->stream, dcur, n, N,
+                       KSPLIT_ROWS, m->part);
+    hipLaunchKernelGGL(reduce_kernel, dim3((N + 255) / 256), dim3(256), 0, m->stream, m->part, nsplit, (int64_t)N,
                        m->g + LAYER_B[l]);
     HIPCHECK(hipGetLastError());
     if (l == 0) break;

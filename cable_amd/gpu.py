@@ -32,8 +32,8 @@ EXPORTS = [
     "cable_gpu_set_class_tables", "cable_gpu_set_forcing", "cable_gpu_run",
     "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters", "cable_gpu_set_tile_order",
     "cable_gpu_run_capture", "cable_gpu_samples", "cable_gpu_copy_samples", "cable_gpu_set_surrogate",
-    "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters",
-    "cable_gpu_last_error",
+    "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
+    "cable_gpu_download_forcing", "cable_gpu_last_error",
 ]
 
 _lib = None
@@ -81,6 +81,9 @@ def lib():
         "cable_gpu_set_surrogate": ([vp, vp, vp], c_int),
         "cable_gpu_run_hybrid": ([vp, c_int, c_int, c_double, c_int, c_double, c_int], c_int),
         "cable_gpu_hybrid_counters": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], c_int),
+        "cable_gpu_synth_forcing": ([vp, c_int, c_int, ctypes.c_uint64, c_double, c_double, c_double, vp, vp],
+                                    c_int),
+        "cable_gpu_download_forcing": ([vp, c_int, c_int, vp, c_int], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -222,6 +225,28 @@ class CableGPU:
     def run(self, k0: int, nsteps: int, dels: float):
         _check(lib().cable_gpu_run(self.h, int(k0), int(nsteps), float(dels)))
 
+    def synth_forcing(self, k_first: int, nsteps: int, seed: int, t0, lat, dels=1800.0, doy=172.0, hod0=0.0):
+        """Generate the synthetic forcing of steps k_first.. on the device
+        (workload.synth_forcing is the same generator on the host)."""
+        t0 = np.ascontiguousarray(t0, dtype=np.float64)
+        lat = np.ascontiguousarray(lat, dtype=np.float64)
+        if t0.shape != (self.mp,) or lat.shape != (self.mp,):
+            raise CableError("t0 and lat need one value per tile")
+        _check(lib().cable_gpu_synth_forcing(self.h, int(k_first), int(nsteps), int(seed) & (2 ** 64 - 1),
+                                             float(dels), float(doy), float(hod0), _ptr(t0), _ptr(lat)))
+        self.nsteps = int(nsteps)
+
+    FORCING = ["met%fsd", "met%fld", "met%tk", "met%qv", "met%ua", "met%pmb", "met%precip", "met%coszen"]
+
+    def download_forcing(self, k: int):
+        """The resident forcing of step k as {name: array(mp)} (met%fsd: band 1)."""
+        out = {}
+        for i, name in enumerate(self.FORCING):
+            a = np.empty(self.mp)
+            _check(lib().cable_gpu_download_forcing(self.h, int(k), i, _ptr(a), HOST_F64))
+            out[name] = a
+        return out
+
     # -- surrogate (mixed contexts) --------------------------------------------
     def run_capture(self, k0: int, nsteps: int, dels: float, capacity: int):
         """Resident steps that also record one training sample per vegetated
@@ -288,5 +313,6 @@ def from_workload(wl, precision=PREC_FP64, full=True, order=None):
         stab = dict(wl.stab)
         stab["zse"] = p["soil%zse"]
         g.set_class_tables(*class_tables(wl.vtab, stab))
-        g.set_forcing(wl.forcing)
+        if getattr(wl, "forcing", None) is not None:   # else: synth_forcing
+            g.set_forcing(wl.forcing)
     return g

@@ -194,12 +194,16 @@ def sample_stats(g, n, chunks=32, rows=16384):
     return norm_stats(np.concatenate(xs), np.concatenate(ys))
 
 
-def train_on_gpu(n_tiles=262144, nsteps=64, epochs=2.0, batch=2048, lr=2e-3, seed=0, log=None):
+def train_on_gpu(n_tiles=262144, nsteps=64, epochs=2.0, batch=2048, lr=2e-3, seed=0, log=None,
+                 max_samples=None):
     """Capture samples with the plain kernel and train the network on them
-    with the GPU kernels.  Returns (params, norm, info)."""
+    with the GPU kernels (on the first ``max_samples`` samples, in step
+    order, when given).  Returns (params, norm, info)."""
     import time
     t0 = time.perf_counter()
     g, n = capture_samples(n_tiles, nsteps)
+    if max_samples is not None:
+        n = min(n, int(max_samples))
     t1 = time.perf_counter()
     norm = sample_stats(g, n)
     net = MLP(batch=batch, params=init_params(seed), norm=norm)
@@ -228,9 +232,10 @@ def main(argv=None):
     ap.add_argument("--tiles", type=int, default=262144)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--epochs", type=float, default=2.0)
+    ap.add_argument("--max-samples", type=int, default=None)
     ap.add_argument("--out", default=None, help="save params and norm (numpy .npz)")
     args = ap.parse_args(argv)
-    params, norm, info = train_on_gpu(args.tiles, args.steps, args.epochs,
+    params, norm, info = train_on_gpu(args.tiles, args.steps, args.epochs, max_samples=args.max_samples,
                                       log=lambda m: print(m, file=sys.stderr, flush=True))
     if args.out:
         np.savez(args.out, params=params, norm=norm)

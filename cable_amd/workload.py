@@ -205,8 +205,7 @@ class Workload:
         self.albsoil[1] = 2.0 * self.albsoil[0]
         self._params()
         self._state0()
-        if forcing:
-            self.forcing = self.make_forcing(0, nsteps)
+        self.forcing = self.make_forcing(0, nsteps) if forcing else None
 
     # ------------------------------------------------------------------
     def _params(self):
@@ -356,3 +355,67 @@ def add_snow(wl, every=3, deep=150.0, thin=5.0):
     sn[1::every] = thin
     wl.state0["ssnow%snowd"] = sn
     return wl
+
+
+# ----------------------------------------------------------------------
+# synthetic forcing keyed by (seed, draw, step, tile): the host form of the
+# device generator behind cable_gpu_synth_forcing (include/cable_gpu.h)
+
+_M64 = (1 << 64) - 1
+
+
+def mix64(z):
+    """SplitMix64's increment and finaliser (Steele, Lea and Flood, OOPSLA
+    2014) on Python ints or numpy uint64 arrays (wrapping arithmetic)."""
+    if isinstance(z, np.ndarray):
+        with np.errstate(over="ignore"):
+            z = z + np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            return z ^ (z >> np.uint64(31))
+    z = (int(z) + 0x9E3779B97F4A7C15) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def synth_keys(seed):
+    """Key of each of the 8 draws (sw, lw, rh, ua Box-Muller pair, rain
+    occurrence, rain amount, psurf)."""
+    return [mix64(mix64(int(seed) & _M64) ^ (s + 1)) for s in range(8)]
+
+
+def synth_u(key, tiles, steps):
+    """U(0,1), open at both ends: (steps x tiles) array."""
+    t = np.asarray(tiles, dtype=np.uint64)[None, :]
+    k = np.asarray(steps, dtype=np.uint64)[:, None]
+    h = mix64(np.uint64(key) ^ (k << np.uint64(32)) ^ t)
+    return ((h >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+
+
+def synth_forcing(t0, lat, k_first, nsteps, seed=DEFAULT_SEED, dels=DELS, doy=DOY, hod0=0.0, tiles=None):
+    """Forcing of steps k_first.. for tiles with mean air temperature t0 (K)
+    and latitude lat (degrees), {name: array(nsteps, n)}, in the device
+    generator's formulas; ``tiles``: the caller's tile indices (default
+    0..n-1), which key the draws."""
+    t0 = np.asarray(t0, dtype=np.float64)
+    lat = np.asarray(lat, dtype=np.float64)
+    n = t0.size
+    tiles = np.arange(n) if tiles is None else np.asarray(tiles)
+    ks = np.arange(k_first, k_first + nsteps)
+    K = synth_keys(seed)
+    u = [synth_u(K[s], tiles, ks) for s in range(8)]
+    hod = (hod0 + ks * (dels / 3600.0))[:, None]
+    two_pi = 2.0 * math.pi
+    diurnal = np.maximum(0.0, 1000.0 * np.sin(two_pi * (hod - 6.0) / 24.0))
+    sw = diurnal * (0.5 + 0.5 * u[0])
+    tair = t0[None, :] + 5.0 * np.sin(two_pi * (hod - 9.0) / 24.0)
+    ps = 950.0 + 70.0 * u[7]
+    qv = rh_sh(100.0 * (0.3 + 0.6 * u[2]), tair, ps)
+    z = np.sqrt(-2.0 * np.log(u[3])) * np.cos(two_pi * u[4])
+    sigma = 0.5
+    ua = np.exp((math.log(3.0) - 0.5 * sigma * sigma) + sigma * z)
+    rain = np.where(u[5] < 0.1, 2.0 * -np.log(u[6]) * (dels / 3600.0), 0.0)
+    cz = np.stack([sinbet(doy, lat, h) for h in hod[:, 0]])
+    return {"met%fsd": 0.5 * sw, "met%fld": 250.0 + 170.0 * u[1], "met%tk": tair, "met%qv": qv,
+            "met%ua": ua, "met%pmb": ps, "met%precip": rain, "met%coszen": cz}

@@ -128,6 +128,28 @@ int cable_gpu_set_forcing(cable_gpu_ctx *ctx, int nsteps, const void *fsd1, cons
                           const void *precip, const void *coszen, int host_type, double ca,
                           double doy);
 
+/* Synthetic forcing generated on the device for steps k_first ..
+   k_first+nsteps-1, replacing the resident forcing (runs of more tiles and
+   steps than a host-side forcing file would hold; no counterpart in the
+   reference, which reads met files in get_met_data,
+   src/offline/cable_input.F90:1650-2692).  The met variables follow the
+   forcing reader: SWdown split evenly over the two bands (:1880-1883),
+   coszen by sinbet (:2677) and, in the step, tvair = tvrad = tk and
+   precip_sn = precip where tk <= 273.16 (:2666-2680), ca = 400 ppm.
+   Per (caller tile t, step k) the draws are U(0,1) values keyed by
+   SplitMix64 on (seed, draw, k, t): SWdown = max(0, 1000 sin(2 pi (hod-6)/24))
+   x U(0.5, 1); LWdown ~ U(250, 420); Tair = t0[t] + 5 sin(2 pi (hod-9)/24);
+   RH ~ U(30, 90)% -> qv by rh_sh (src/offline/cable_checks.F90:633-696);
+   wind lognormal with mean 3 m/s and sigma 0.5; precip Bernoulli(0.1) x
+   Exp(mean 2 mm/h) in mm per step; PSurf ~ U(950, 1020) hPa; hod = hod0 +
+   k dels/3600; lat[t] in degrees.  t0 and lat are the caller's per-tile
+   arrays (mp values).  cable_gpu_run then accepts steps in the window. */
+int cable_gpu_synth_forcing(cable_gpu_ctx *ctx, int k_first, int nsteps, uint64_t seed, double dels,
+                            double doy, double hod0, const double *t0, const double *lat);
+/* One resident forcing variable of step k in the caller's tile order;
+   which: 0 fsd(:,1), 1 fld, 2 tk, 3 qv, 4 ua, 5 pmb, 6 precip, 7 coszen. */
+int cable_gpu_download_forcing(cable_gpu_ctx *ctx, int k, int which, void *host, int host_type);
+
 /* Run steps k0..k0+nsteps-1 of the resident forcing (ktau = k0+1 ...),
    writing only the prognostic state and the eight headline fluxes
    (canopy%rnet, fe, fh, fev, fes, fgpp, ssnow%runoff, rad%trad).  The
