@@ -59,7 +59,7 @@ def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None, ke
     wl = Workload(n, nsteps)
     log(f"[bench] workload {n} tiles x {nsteps} steps generated in {time.perf_counter() - t:.1f}s")
     prec = gpu.PREC_FP64 if precision == "fp64" else gpu.PREC_MIXED
-    order = gpu.class_order(wl.iveg, wl.isoilm) if class_order else None
+    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai) if class_order else None
 
     def go(g, k):
         if hybrid is None:
@@ -148,7 +148,7 @@ def production(args, log):
     wl = Workload(n, 1, forcing=False)
     log(f"[bench] production workload {n} tiles generated in {time.perf_counter() - t:.1f}s")
     params, norm, info = surrogate.train_on_gpu(log=log, **TRAIN)
-    order = gpu.class_order(wl.iveg, wl.isoilm)
+    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai)
     ctx = {"fp64": gpu.from_workload(wl, precision=gpu.PREC_FP64, full=False, order=order),
            "mixed": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order),
            "hybrid": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order)}
@@ -203,7 +203,7 @@ def production(args, log):
         "data": "synthetic (forcing generated on the device, parameters from the reference's PFT and soil tables)",
         "config": {"workload": f"hybrid production run, {n} tiles x {total} half-hourly steps",
                    "tiles": n, "precision": "mixed + BF16 surrogate warm start", "path": "device-resident run",
-                   "tile_order": "grouped by vegetation and soil type", "forcing_window_steps": win},
+                   "tile_order": "grouped by vegetation and soil type, then leaf area", "forcing_window_steps": win},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,
@@ -300,7 +300,7 @@ def main():
         "data": "synthetic (seeded forcing, parameters from the reference's PFT and soil tables)",
         "config": {"workload": f"{what}, {args.tiles} tiles x {args.steps} half-hourly steps",
                    "tiles": args.tiles, "precision": prec, "path": "device-resident run",
-                   "tile_order": "caller" if args.caller_order else "grouped by vegetation and soil type"},
+                   "tile_order": "caller" if args.caller_order else "grouped by vegetation and soil type, then leaf area"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src},

@@ -289,13 +289,16 @@ class CableGPU:
         _check(lib().cable_gpu_sync(self.h))
 
 
-def class_order(iveg, isoilm=None):
-    """A tile order that groups tiles by vegetation type (then soil type):
-    the wavefronts then run one type's canopy iteration counts and branches."""
+def class_order(iveg, isoilm=None, vlai=None):
+    """A tile order that groups tiles by vegetation type (then soil type, then
+    leaf area): the wavefronts then run one type's canopy iteration counts and
+    branches, and tiles of similar leaf area side by side."""
     iveg = np.asarray(iveg)
     if isoilm is None:
         return np.argsort(iveg, kind="stable")
-    return np.lexsort((np.asarray(isoilm), iveg))
+    if vlai is None:
+        return np.lexsort((np.asarray(isoilm), iveg))
+    return np.lexsort((np.asarray(vlai), np.asarray(isoilm), iveg))
 
 
 def from_workload(wl, precision=PREC_FP64, full=True, order=None):
