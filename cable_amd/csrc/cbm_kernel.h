@@ -423,7 +423,10 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
 // launch one step over mp tiles on `stream` (256 threads per workgroup)
 template <class R, bool FULL, bool TABLES, int MODE = MODE_PLAIN>
 hipError_t launch_cbm(const StepArgs &a, int64_t mp, hipStream_t stream) {
-  const int block = 256;
+  // FP64 runs one wave per SIMD: one-wave workgroups let the dispatcher fill
+  // SIMDs wave by wave (measured 2-4% faster than four-wave workgroups); the
+  // FP32 kernels keep four waves per workgroup, two resident per SIMD
+  const int block = sizeof(R) == 8 ? 64 : 256;
   const int64_t grid = (mp + block - 1) / block;
   hipLaunchKernelGGL((cbm_kernel<R, FULL, TABLES, MODE>), dim3((unsigned)grid), dim3(block), 0, stream, a);
   return hipGetLastError();
