@@ -178,13 +178,11 @@ cbm_kernel(StepArgs a) {
       T.ssnow_isflag[0] = ld<int>(F[CABLE_F_ssnow_isflag], t);
       LDS_(ssnow_pudsto, 1) LDS_(ssnow_rtsoil, 1) LDS_(ssnow_wbtot1, 1) LDS_(ssnow_wbtot2, 1)
       LDS_(ssnow_wb_lake, 1) LDS_(ssnow_owetfac, 1) LDS_(ssnow_snage, 1) LDS_(ssnow_snowd, 1)
-      LDS_(ssnow_ssdnn, 1) LDS_(ssnow_tss, 1) LDS_(ssnow_smass, 3) LDS_(ssnow_ssdn, 3)
-      LDS_(ssnow_tgg, 6) LDS_(ssnow_tggsn, 3) LDS_(canopy_cansto, 1) LDS_(bgc_cplant, 3)
-      LDS_(bgc_csoil, 2)
-      LDD_(ssnow_wb, 6) LDD_(ssnow_wbice, 6) LDD_(drv_owb, 1)
-      LDD_(bal_ebal_tot, 1) LDD_(bal_wbal_tot, 1) LDD_(bal_precip_tot, 1) LDD_(bal_rnoff_tot, 1)
-      LDD_(bal_evap_tot, 1) LDD_(bal_radbalsum, 1)
-      T.ssnow_gammzz[0] = ld<double>(F[CABLE_F_ssnow_gammzz], t);
+      LDS_(ssnow_ssdnn, 1) LDS_(ssnow_tss, 1)
+      LDS_(ssnow_tgg, 6) LDS_(ssnow_tggsn, 3) LDS_(canopy_cansto, 1)
+      LDD_(ssnow_wb, 6) LDD_(ssnow_wbice, 6)
+      // snow layers, carbon pools, gammzz(:,1) and the balance accumulators:
+      // loaded after the canopy (Stepper::run's `late`)
       // soil liquid water as the previous step left it (wb - wbice), before
       // this step's lake fix-up
 #pragma unroll
@@ -223,6 +221,22 @@ cbm_kernel(StepArgs a) {
     const double *vrow = TABLES ? lds_v + (iv < 0 ? 0 : iv) * VC_N : nullptr;
     const double *srow = TABLES ? lds_s + (is < 0 ? 0 : is) * SC_N : nullptr;
     Stepper<R, FULL, TABLES, MODE> S(T, a, vrow, srow);
+    auto late = [&]() {
+      if constexpr (!FULL) {
+        void *const *F = a.F;
+        using DT = typename DevT<R, 'F'>::type;
+#define LDS_(id, n) \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+#define LDD_(id, n) \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<double>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+        LDS_(ssnow_smass, 3) LDS_(ssnow_ssdn, 3) LDS_(bgc_cplant, 3) LDS_(bgc_csoil, 2)
+        LDD_(drv_owb, 1) LDD_(bal_ebal_tot, 1) LDD_(bal_wbal_tot, 1) LDD_(bal_precip_tot, 1)
+        LDD_(bal_rnoff_tot, 1) LDD_(bal_evap_tot, 1) LDD_(bal_radbalsum, 1)
+        T.ssnow_gammzz[0] = ld<double>(F[CABLE_F_ssnow_gammzz], t);
+#undef LDS_
+#undef LDD_
+      }
+    };
     if constexpr (MODE == MODE_CAPTURE) surrogate_features<R>(a, t, vrow, srow, feat);
     if constexpr (MODE == MODE_HYBRID) {
       if (a.pred) {
@@ -230,7 +244,7 @@ cbm_kernel(StepArgs a) {
         for (int c = 0; c < mlp::NOUT; c++) S.pred[c] = a.pred[t * mlp::NOUT + c];
       }
     }
-    S.run();
+    S.run(late);
     passes = S.npass;
     if constexpr (MODE == MODE_HYBRID) fallback = S.vegetated && S.resid > a.resid_max;
     if constexpr (MODE == MODE_CAPTURE) {

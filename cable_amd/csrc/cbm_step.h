@@ -1955,6 +1955,14 @@ struct Stepper {
   // 725-729) and the balance accumulators (src/offline/cable_checks.F90
   // :472-618) in FP64
   CBM_HD void run() {
+    run([] {});
+  }
+  // `late` runs after the canopy and before soil and snow: the resident
+  // kernel loads there the state that only soil, snow, carbon and the
+  // balances read, so it is not held in registers through the canopy
+  // iteration
+  template <class Late>
+  CBM_HD void run(Late &&late) {
     R dels = (R)a.dels;
     if (a.driver) {
       // the offline driver before cbm, src/offline/cable_serial.F90:572-575
@@ -1978,6 +1986,7 @@ struct Stepper {
     t.ssnow_otss_0[0] = t.ssnow_otss[0];
     t.ssnow_otss[0] = t.ssnow_tss[0];
     define_canopy(sunlit_veg);
+    late();
     t.ssnow_owetfac[0] = t.ssnow_wetfac[0];
     soil_snow();
     snow_aging();
