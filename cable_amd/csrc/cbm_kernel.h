@@ -179,14 +179,17 @@ cbm_kernel(StepArgs a) {
       LDS_(ssnow_pudsto, 1) LDS_(ssnow_rtsoil, 1) LDS_(ssnow_wbtot1, 1) LDS_(ssnow_wbtot2, 1)
       LDS_(ssnow_wb_lake, 1) LDS_(ssnow_owetfac, 1) LDS_(ssnow_snage, 1) LDS_(ssnow_snowd, 1)
       LDS_(ssnow_ssdnn, 1) LDS_(ssnow_tss, 1)
-      LDS_(ssnow_tgg, 6) LDS_(ssnow_tggsn, 3) LDS_(canopy_cansto, 1)
-      LDD_(ssnow_wb, 6) LDD_(ssnow_wbice, 6)
-      // snow layers, carbon pools, gammzz(:,1) and the balance accumulators:
-      // loaded after the canopy (Stepper::run's `late`)
-      // soil liquid water as the previous step left it (wb - wbice), before
-      // this step's lake fix-up
+      LDS_(ssnow_tgg, 1) LDS_(ssnow_tggsn, 1) LDS_(canopy_cansto, 1)
+      LDD_(ssnow_wb, 1) LDD_(ssnow_wbice, 1)
+      // the canopy reads only the top soil and snow layer; the deeper layers,
+      // snow layers, carbon pools, gammzz(:,1) and the balance accumulators
+      // are loaded after the canopy (Stepper::run's `late`).  Soil liquid
+      // water as the previous step left it (wb - wbice), before this step's
+      // lake fix-up:
 #pragma unroll
-      for (int k = 0; k < MS; k++) T.ssnow_wbliq[k] = T.ssnow_wb[k] - T.ssnow_wbice[k];
+      for (int k = 0; k < MS; k++)
+        T.ssnow_wbliq[k] = ld<double>(F[CABLE_F_ssnow_wb], (int64_t)k * mp + t) -
+                           ld<double>(F[CABLE_F_ssnow_wbice], (int64_t)k * mp + t);
       // per-tile parameters
       T.veg_iveg[0] = ld<int>(F[CABLE_F_veg_iveg], t);
       T.soil_isoilm[0] = ld<int>(F[CABLE_F_soil_isoilm], t);
@@ -229,6 +232,14 @@ cbm_kernel(StepArgs a) {
   { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t); }
 #define LDD_(id, n) \
   { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<double>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+        // a fresh read: nothing has written these since the gather, and the
+        // compiler must not keep the gather's values live to reuse them
+        __asm__ __volatile__("" ::: "memory");
+#define LDL_(T_, id, k0, n) \
+  { _Pragma("unroll") for (int k = k0; k < n; k++) T.id[k] = ld<T_>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+        LDL_(DT, ssnow_tgg, 1, 6) LDL_(DT, ssnow_tggsn, 1, 3) LDL_(double, ssnow_wb, 1, 6)
+        LDL_(double, ssnow_wbice, 1, 6)
+#undef LDL_
         LDS_(ssnow_smass, 3) LDS_(ssnow_ssdn, 3) LDS_(bgc_cplant, 3) LDS_(bgc_csoil, 2)
         LDD_(drv_owb, 1) LDD_(bal_ebal_tot, 1) LDD_(bal_wbal_tot, 1) LDD_(bal_precip_tot, 1)
         LDD_(bal_rnoff_tot, 1) LDD_(bal_evap_tot, 1) LDD_(bal_radbalsum, 1)
