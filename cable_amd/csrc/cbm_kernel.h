@@ -27,13 +27,17 @@ using cbm_mlp::v4s;
 enum { ROLE_PARAM = 0, ROLE_FORC = 1, ROLE_STATE = 2, ROLE_DIAG = 3 };
 
 
+// field arrays are device (global) memory: global loads and stores rather
+// than flat ones, which also count against the LDS/scalar wait counter
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
 template <class T>
 __device__ __forceinline__ T ld(const void *p, int64_t i) {
-  return static_cast<const T *>(p)[i];
+  return ((const gptr<const T>)p)[i];
 }
 template <class T, class V>
 __device__ __forceinline__ void st(void *p, int64_t i, V v) {
-  static_cast<T *>(p)[i] = static_cast<T>(v);
+  ((gptr<T>)p)[i] = static_cast<T>(v);
 }
 
 // device type of a field of precision code P for working precision R
