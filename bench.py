@@ -95,6 +95,47 @@ def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None, ke
                  "kernel_ms": ev_ms / steps, "achieved_GBs": achieved, "dryleaf_passes": passes}, **extra)
 
 
+def dropin_form(n, steps, log):
+    """The drop-in step as the Fortran cbm replacement drives it: every field
+    uploaded from host arrays, one cable_gpu_step, every state and diagnostic
+    field downloaded -- the PCIe-inclusive rate of the host-buffer boundary."""
+    from cable_amd import gpu
+    from cable_amd.fields import FIELDS
+    from cable_amd.workload import Workload
+    wl = Workload(n, steps)
+    host = {f.name: np.zeros(f.shape(n)) for f in FIELDS}
+    for d in (wl.params, wl.state0):
+        for k, v in d.items():
+            if k in host:
+                host[k] = np.asarray(v, dtype=np.float64).reshape(host[k].shape)
+    g = gpu.CableGPU(n, gpu.PREC_FP64)
+    p = wl.params
+    g.set_globals(p["soil%zse"], p["soil%zshh"], p["bgc%ratecp"], p["bgc%ratecs"])
+    up = dn = dev = 0.0
+    for k in range(steps):
+        for name, v in wl.forcing.items():
+            if name in host:
+                host[name] = np.asarray(v[k], dtype=np.float64).reshape(host[name].shape)
+        t0 = time.perf_counter()
+        for f in FIELDS:
+            g.upload(f.name, host[f.name])
+        t1 = time.perf_counter()
+        g.step(k + 1, wl.dels)
+        g.sync()
+        t2 = time.perf_counter()
+        for f in FIELDS:
+            if f.role in ("STATE", "DIAG"):
+                host[f.name] = g.download(f.name)
+        t3 = time.perf_counter()
+        up, dev, dn = up + t1 - t0, dev + t2 - t1, dn + t3 - t2
+    g.close()
+    wall = up + dev + dn
+    log(f"[bench] drop-in fp64 n={n}: {wall / steps * 1e3:.1f} ms per step "
+        f"(upload {up / steps * 1e3:.1f}, step {dev / steps * 1e3:.1f}, download {dn / steps * 1e3:.1f})")
+    return {"tiles": n, "steps": steps, "value": n * steps / wall, "ms_per_step": wall * 1e3 / steps,
+            "upload_ms": up * 1e3 / steps, "step_ms": dev * 1e3 / steps, "download_ms": dn * 1e3 / steps}
+
+
 def profiled_traffic(tiles, precision):
     """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
     scaled by a plain copy) for this workload, from the committed profile
@@ -278,6 +319,7 @@ def main():
         hyb["flux_error_vs_fp64_after_run"] = flux_errors(hyb.pop("_out"), ref)
         hyb["surrogate_training"] = info
         hyb["max_passes"], hyb["resid_max_Wm2"] = 4, 0.5
+        forms["dropin_fp64_262144"] = dropin_form(262144, 3, log)
         forms["fp64_1048576"] = f64
         forms["mixed_1048576"] = mix
         forms["hybrid_1048576"] = hyb
