@@ -169,9 +169,33 @@ def cpu_baseline(steps, log):
         kind, what = "port", "FP64 C restatement of the reference cbm (gcc -O2)"
     v = CPU_SAMPLE_TILES * steps / sec
     log(f"[bench] cpu baseline ({kind}): {sec:.2f}s, {v:.4g} tile-steps/s")
-    return {"value": v, "unit": "tile-steps/s", "cores": 1, "kind": kind,
-            "sample": f"first {CPU_SAMPLE_TILES} tiles of the workload x {steps} steps; {what}; "
-                      f"one process; {sec:.2f} s"}
+    out = {"value": v, "unit": "tile-steps/s", "cores": 1, "kind": kind,
+           "sample": f"first {CPU_SAMPLE_TILES} tiles of the workload x {steps} steps; {what}; "
+                     f"one process; {sec:.2f} s"}
+    if kind == "reference":
+        out.update(_cpu_reference_more(wl, steps, log))
+    return out
+
+
+def _cpu_reference_more(wl, steps, log):
+    """The shipped-precision reference on one core, and the promoted one on
+    all host cores (one process per contiguous block of tiles, as the MPI
+    workers split land points; cbm keeps module state, so no threads)."""
+    import concurrent.futures as cf
+    from oracle import refrun
+    more = {}
+    if refrun.available("r4"):
+        _, sec = refrun.run(wl, steps, [], every_step=False, build="r4")
+        more["shipped_precision_one_core"] = {"value": CPU_SAMPLE_TILES * steps / sec, "seconds": sec}
+        log(f"[bench] cpu baseline shipped precision: {sec:.2f}s")
+    p = max(1, min(16, os.cpu_count() or 1))
+    blocks = np.array_split(np.arange(CPU_SAMPLE_TILES), p)
+    with cf.ThreadPoolExecutor(p) as ex:
+        secs = list(ex.map(lambda b: refrun.run(wl.subset(b), steps, [], every_step=False, build="r8")[1], blocks))
+    more["all_cores"] = {"value": CPU_SAMPLE_TILES * steps / max(secs), "processes": p,
+                         "seconds": max(secs)}
+    log(f"[bench] cpu baseline on {p} processes: {max(secs):.2f}s")
+    return more
 
 
 TRAIN = {"n_tiles": 262144, "nsteps": 96, "epochs": 2.0, "max_samples": 16777216}
