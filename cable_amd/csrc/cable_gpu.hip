@@ -733,7 +733,7 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
     a.qlen = c->qlen;
     HIPCHECK(hipMemsetAsync(c->qlen, 0, sizeof(unsigned), c->stream));
     hipError_t e = hipSuccess;
-    if (use_surrogate) e = launch_mlp_infer(a, c->swt, c->sparams, c->pred, c->mp, c->stream);
+    if (use_surrogate) e = launch_mlp_infer(a, c->swt, c->sparams, c->pred, nullptr, c->mp, c->stream);
     if (e == hipSuccess) e = launch_mixed_mode(MODE_HYBRID, a, c->mp, c->stream);
     // the tiles the capped iteration did not settle: the plain method, in
     // place, from the state the hybrid step left untouched for them
@@ -758,6 +758,37 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
   c->last_passes = (int64_t)np;
   c->last_fallbacks = (int64_t)fb;
   c->last_tile_steps = (int64_t)nsteps * c->mp;
+  return CABLE_OK;
+}
+
+int cable_gpu_surrogate_eval(cable_gpu_ctx *c, int k, float *x, float *pred) {
+  int r = check_resident(c, k, 1, 1.0, "cable_gpu_surrogate_eval");
+  if (r) return r;
+  if (!c->swt) return fail(CABLE_E_STATE, "cable_gpu_surrogate_eval: no surrogate set");
+  if (!x || !pred) return fail(CABLE_E_ARG, "cable_gpu_surrogate_eval: null output");
+  using namespace cbm_mlp;
+  const size_t mp = (size_t)c->mp;
+  float *dx = nullptr, *dp = nullptr;
+  HIPCHECK(hipMalloc(&dx, sizeof(float) * NIN * mp));
+  HIPCHECK(hipMalloc(&dp, sizeof(float) * NOUT * mp));
+  StepArgs a = make_args(c, c->d_tab_run, k + 1, 1.0, (int64_t)(k - c->forc_k0) * c->mp);
+  a.hyb = c->hyb;
+  a.hyb_valid = (c->hyb && c->hyb_valid) ? 1 : 0;
+  a.norm = c->snorm;
+  hipError_t e = launch_mlp_infer(a, c->swt, c->sparams, dp, dx, c->mp, c->stream);
+  if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_surrogate_eval launch: %s", hipGetErrorString(e));
+  std::vector<float> hx(NIN * mp), hp(NOUT * mp);
+  HIPCHECK(hipMemcpyAsync(hx.data(), dx, sizeof(float) * hx.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(hp.data(), dp, sizeof(float) * hp.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipFree(dx));
+  HIPCHECK(hipFree(dp));
+  // rows back to the caller's tile order
+  for (size_t j = 0; j < mp; j++) {
+    const size_t d = c->order.empty() ? j : (size_t)c->order[j];
+    memcpy(x + d * NIN, hx.data() + j * NIN, sizeof(float) * NIN);
+    memcpy(pred + d * NOUT, hp.data() + j * NOUT, sizeof(float) * NOUT);
+  }
   return CABLE_OK;
 }
 

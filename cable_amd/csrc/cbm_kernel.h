@@ -341,108 +341,174 @@ cbm_kernel(StepArgs a) {
 
 // ---------------------------------------------------------------------------
 // surrogate inference for every tile of a step: 24 features -> network ->
-// 8 denormalised predictions in pred[t*8 ..].  Four waves per workgroup, 32
-// tiles per wave; activations in LDS as BF16, weights read in their
-// transposed BF16 layout wt (per layer [out][in]) so each lane's four
-// K-consecutive operands are one 8-byte load; biases FP32 from `params`.
-template <int NWAVES>
+// 8 denormalised predictions in pred[t*8 ..] (and, when xraw is given, the
+// raw features in xraw[t*24 ..]).
+//
+// One workgroup per CU of eight waves steps blocks of 256 tiles, 32 per
+// wave, two waves per SIMD so that one wave's matrix-core products overlap
+// the other's SiLU.  The weights (BF16, transposed [out][in], rows padded by
+// 8) and biases are staged once per workgroup into LDS; each wave keeps its 32 activation rows
+// in LDS between layers.  A layer is computed transposed, D = W^T x A^T on
+// v_mfma_f32_32x32x16_bf16: a lane then holds four consecutive outputs of
+// one tile per register group, which it writes back as one 8-byte row
+// segment.  Layer 1's K (24) is padded with zeros to 32.
+namespace infer {
+using namespace mlp;
+constexpr int WAVES = 8, TILES = 32 * WAVES;
+constexpr int K1 = 32;                           // layer-1 K padded to a multiple of 16
+constexpr int P1 = K1 + 8, PH = HID + 8;         // weight row pitches (BF16)
+constexpr int AP = HID + 8;                      // activation row pitch
+constexpr int OFF1 = 0, OFF2 = OFF1 + HID * P1, OFF3 = OFF2 + HID * PH, OFF4 = OFF3 + HID * PH;
+constexpr int WLDS = OFF4 + NOUT * PH;           // 41,024 BF16
+constexpr int ALDS = WAVES * 32 * AP;            // 34,816 BF16
+constexpr int BLDS = 3 * HID + NOUT;             // biases, FP32
+static_assert(2 * (WLDS + ALDS) + 4 * BLDS <= 160 * 1024, "fits the 160 KB of LDS");
+// the transposed global layout: per layer [out][in], layers back to back
+constexpr int G1 = 0, G2 = G1 + HID * NIN, G3 = G2 + HID * HID, G4 = G3 + HID * HID, GTOT = G4 + NOUT * HID;
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+__device__ inline float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+}  // namespace infer
+
+// (a template so that the header's kernel has one definition across the
+// translation units that include it)
+template <int NWAVES = infer::WAVES>
 __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, const uint16_t *__restrict__ wt,
-                                                        const float *__restrict__ params,
-                                                        float *__restrict__ pred) {
+                                                               const float *__restrict__ params,
+                                                               float *__restrict__ pred, float *__restrict__ xraw) {
   using namespace mlp;
-  // rows padded by 4 and 8 BF16 so that the 32 rows a wave reads at one K
-  // offset fall in different LDS banks
-  __shared__ uint16_t xs[NWAVES][32][NIN + 4];
-  __shared__ uint16_t hs[NWAVES][32][HID + 4];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = lane & 31, kq = 4 * (lane >> 5);
-  const int64_t base = ((int64_t)blockIdx.x * NWAVES + wave) * 32;
-  const float *norm = a.norm;
-  // a workgroup whose 128 tiles carry no leaves has nothing to predict (the
-  // warm start applies to vegetated tiles only); the decision is uniform
-  // over the workgroup, so it leaves before any barrier
+  using namespace infer;
+  static_assert(NWAVES == WAVES, "LDS sized for WAVES");
+  __shared__ __attribute__((aligned(16))) __bf16 wl[WLDS];
+  __shared__ __attribute__((aligned(16))) __bf16 act[ALDS];
+  __shared__ float bl[BLDS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31;
+  const int kh = 8 * (lane >> 5), kq = 4 * (lane >> 5);
+  // stage the weights once per workgroup (the grid is one workgroup per CU,
+  // each stepping every gridDim.x-th block of 256 tiles): 8-byte units of
+  // four consecutive inputs, loads issued together before the stores
   {
-    const int64_t t = (int64_t)blockIdx.x * NWAVES * 32 + threadIdx.x;
-    const bool leafy = t < a.mp && threadIdx.x < NWAVES * 32 && ld<int>(a.F[CABLE_F_veg_iveg], t) < 14;
-    if (!__syncthreads_or(leafy)) return;
-  }
-  if (lane < 32) {
-    const int64_t t = base + lane;
-    float x[NIN];
-    if (t < a.mp) {
-      const int iv = ld<int>(a.F[CABLE_F_veg_iveg], t) - 1, is = ld<int>(a.F[CABLE_F_soil_isoilm], t) - 1;
-      surrogate_features<float>(a, t, a.vtab + (iv < 0 ? 0 : iv) * VC_N, a.stab + (is < 0 ? 0 : is) * SC_N, x);
+    constexpr int U = GTOT / 4, PER = (U + 64 * WAVES - 1) / (64 * WAVES);
+    uint2 v[PER];
 #pragma unroll
-      for (int c = 0; c < NIN; c++) x[c] = (x[c] - norm[c]) / norm[NIN + c];
-    } else {
-#pragma unroll
-      for (int c = 0; c < NIN; c++) x[c] = 0.0f;
+    for (int q = 0; q < PER; q++) {
+      const int u = threadIdx.x + q * 64 * WAVES;
+      if (u < U) v[q] = *(const uint2 *)&wt[4 * u];
     }
 #pragma unroll
-    for (int c = 0; c < NIN; c++) xs[wave][lane][c] = f32_to_bf16(x[c]);
-  }
-  __syncthreads();
-  // hidden layers
-  int woff = 0;
-  for (int l = 0; l < 3; l++) {
-    const int K = LAYER_IN[l];
-    const uint16_t *W = wt + woff;
-    const float *bias = params + LAYER_B[l];
-    v16f acc[4];
-#pragma unroll
-    for (int nb = 0; nb < 4; nb++)
-#pragma unroll
-      for (int r = 0; r < 16; r++) acc[nb][r] = 0.0f;
-    // K outer: one A operand from LDS feeds the four 32-column blocks, whose
-    // weight operands are loaded together and whose products are independent
-    for (int k0 = 0; k0 < K; k0 += 8) {
-      v4s av, bv[4];
-      const uint16_t *src = (l == 0) ? &xs[wave][row][k0 + kq] : &hs[wave][row][k0 + kq];
-#pragma unroll
-      for (int e = 0; e < 4; e++) av[e] = (short)src[e];
-#pragma unroll
-      for (int nb = 0; nb < 4; nb++) {
-        const uint16_t *wsrc = W + (int64_t)(nb * 32 + row) * K + k0 + kq;
-#pragma unroll
-        for (int e = 0; e < 4; e++) bv[nb][e] = (short)wsrc[e];
+    for (int q = 0; q < PER; q++) {
+      const int u = threadIdx.x + q * 64 * WAVES, e = 4 * u;
+      if (u < U) {
+        int dst;
+        if (e < G2) dst = OFF1 + (e / NIN) * P1 + e % NIN;
+        else if (e < G3) dst = OFF2 + ((e - G2) / HID) * PH + (e - G2) % HID;
+        else if (e < G4) dst = OFF3 + ((e - G3) / HID) * PH + (e - G3) % HID;
+        else dst = OFF4 + ((e - G4) / HID) * PH + (e - G4) % HID;
+        *(uint2 *)&wl[dst] = v[q];
       }
-#pragma unroll
-      for (int nb = 0; nb < 4; nb++) acc[nb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv[nb], acc[nb], 0, 0, 0);
     }
-    __syncthreads();  // every wave has read this layer's input
+  }
+  for (int u = threadIdx.x; u < HID * 2; u += blockDim.x)
+    *(uint2 *)&wl[OFF1 + (u >> 1) * P1 + NIN + 4 * (u & 1)] = make_uint2(0u, 0u);
+  for (int u = threadIdx.x; u < BLDS; u += blockDim.x)
+    bl[u] = params[u < 3 * HID ? LAYER_B[u / HID] + u % HID : LAYER_B[3] + u - 3 * HID];
+  __bf16 *A = &act[wave * 32 * AP];
+  const int64_t nblk = (a.mp + TILES - 1) / TILES;
+  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t t0 = blk * TILES;
+    // a block whose tiles carry no leaves has nothing to predict (the warm
+    // start applies to vegetated tiles only).  The vote is also the barrier
+    // between the previous block's layers and this block's feature rows.
+    {
+      const int64_t t = t0 + threadIdx.x;
+      const bool leafy = threadIdx.x < TILES && t < a.mp && ld<int>(a.F[CABLE_F_veg_iveg], t) < 14;
+      if (!__syncthreads_or(leafy) && !xraw) continue;
+    }
+    // features of tile threadIdx.x, normalised, as BF16 row threadIdx.x
+    if (threadIdx.x < TILES) {
+      const int64_t t = t0 + threadIdx.x;
+      __bf16 *row = &act[threadIdx.x * AP];
+      float x[NIN];
+      if (t < a.mp) {
+        const int iv = ld<int>(a.F[CABLE_F_veg_iveg], t) - 1, is = ld<int>(a.F[CABLE_F_soil_isoilm], t) - 1;
+        surrogate_features<float>(a, t, a.vtab + (iv < 0 ? 0 : iv) * VC_N, a.stab + (is < 0 ? 0 : is) * SC_N, x);
+        if (xraw) {
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) {
-      const int j = nb * 32 + row;
+          for (int c = 0; c < NIN; c++) xraw[t * NIN + c] = x[c];
+        }
 #pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int i = (r & 3) + kq + 8 * (r >> 2);
-        hs[wave][i][j] = f32_to_bf16(silu(acc[nb][r] + bias[j]));
+        for (int c = 0; c < NIN; c++) x[c] = (x[c] - a.norm[c]) / a.norm[NIN + c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < NIN; c++) x[c] = 0.0f;
       }
+#pragma unroll
+      for (int c = 0; c < NIN; c += 4) {
+        bf4 v = {(__bf16)x[c], (__bf16)x[c + 1], (__bf16)x[c + 2], (__bf16)x[c + 3]};
+        *(bf4 *)&row[c] = v;
+      }
+      *(uint2 *)&row[NIN] = make_uint2(0u, 0u);
+      *(uint2 *)&row[NIN + 4] = make_uint2(0u, 0u);
     }
     __syncthreads();
-    woff += K * HID;
-  }
-  // output layer: N = 8 of the 32 columns
-  {
-    const uint16_t *W = wt + woff;
-    const float *bias = params + LAYER_B[3];
-    v16f acc;
+    // from here on a wave touches only its own 32 activation rows: its LDS
+    // operations complete in order, so the layers need no workgroup barrier
+#pragma unroll 1
+    for (int l = 0; l < 3; l++) {
+      const int K = l == 0 ? K1 : HID;
+      const __bf16 *W = &wl[l == 0 ? OFF1 : (l == 1 ? OFF2 : OFF3)];
+      const int pw = l == 0 ? P1 : PH;
+      v16f acc[4];
 #pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-    for (int k0 = 0; k0 < HID; k0 += 8) {
-      v4s av, bv;
+      for (int nb = 0; nb < 4; nb++)
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        av[e] = (short)hs[wave][row][k0 + kq + e];
-        bv[e] = row < NOUT ? (short)W[row * HID + k0 + kq + e] : (short)0;
+        for (int r = 0; r < 16; r++) acc[nb][r] = 0.0f;
+      for (int k0 = 0; k0 < K; k0 += 16) {
+        const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
+#pragma unroll
+        for (int nb = 0; nb < 4; nb++) {
+          const bf8 wv = *(const bf8 *)&W[(nb * 32 + col) * pw + k0 + kh];
+          acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xv, acc[nb], 0, 0, 0);
+        }
       }
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
-    }
-    if (row < NOUT) {
-      const float mu = norm[2 * NIN + row], sd = norm[2 * NIN + NOUT + row];
+      __builtin_amdgcn_wave_barrier();
+      // lane: tile `col`; register group g of block nb: outputs nb*32 + 8g + kq + 0..3
+      const float *b = &bl[l * HID];
 #pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int64_t t = base + (r & 3) + kq + 8 * (r >> 2);
-        if (t < a.mp) pred[t * NOUT + row] = (acc[r] + bias[row]) * sd + mu;
+      for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const int j = nb * 32 + 8 * g + kq;
+          bf4 v;
+#pragma unroll
+          for (int q = 0; q < 4; q++) v[q] = (__bf16)silu_fast(acc[nb][4 * g + q] + b[j + q]);
+          *(bf4 *)&A[col * AP + j] = v;
+        }
+      __builtin_amdgcn_wave_barrier();
+    }
+    // output layer: 8 outputs, rows 8..31 of the weight operand are zero
+    {
+      v16f acc;
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+      for (int k0 = 0; k0 < HID; k0 += 16) {
+        const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
+        bf8 wv;
+        if (col < NOUT) wv = *(const bf8 *)&wl[OFF4 + col * PH + k0 + kh];
+        else
+#pragma unroll
+          for (int e = 0; e < 8; e++) wv[e] = (__bf16)0.0f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xv, acc, 0, 0, 0);
+      }
+      // lane: tile `col`, outputs kq + 0..3 in registers 0..3
+      const int64_t t = t0 + wave * 32 + col;
+      if (t < a.mp) {
+        float4 o;
+        o.x = (acc[0] + bl[3 * HID + kq + 0]) * a.norm[2 * NIN + NOUT + kq + 0] + a.norm[2 * NIN + kq + 0];
+        o.y = (acc[1] + bl[3 * HID + kq + 1]) * a.norm[2 * NIN + NOUT + kq + 1] + a.norm[2 * NIN + kq + 1];
+        o.z = (acc[2] + bl[3 * HID + kq + 2]) * a.norm[2 * NIN + NOUT + kq + 2] + a.norm[2 * NIN + kq + 2];
+        o.w = (acc[3] + bl[3 * HID + kq + 3]) * a.norm[2 * NIN + NOUT + kq + 3] + a.norm[2 * NIN + kq + 3];
+        *(float4 *)&pred[t * NOUT + kq] = o;
       }
     }
   }
@@ -468,7 +534,7 @@ hipError_t launch_mixed(bool full, const StepArgs &a, int64_t mp, hipStream_t st
 // surrogate inference over mp tiles (cbm_kernels_mixed.hip)
 hipError_t launch_mixed_mode(int mode, const StepArgs &a, int64_t mp, hipStream_t stream);
 hipError_t launch_mlp_infer(const StepArgs &a, const uint16_t *wt, const float *params, float *pred,
-                            int64_t mp, hipStream_t stream);
+                            float *xraw, int64_t mp, hipStream_t stream);
 hipError_t launch_count(const unsigned *qlen, unsigned long long *total, hipStream_t stream);
 
 }  // namespace cbm

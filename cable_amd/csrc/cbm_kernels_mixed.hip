@@ -27,10 +27,21 @@ hipError_t launch_count(const unsigned *qlen, unsigned long long *total, hipStre
   return hipGetLastError();
 }
 
-hipError_t launch_mlp_infer(const StepArgs &a, const uint16_t *wt, const float *params, float *pred, int64_t mp,
-                            hipStream_t stream) {
-  const int64_t grid = (mp + 127) / 128;   // 4 waves x 32 tiles per workgroup
-  hipLaunchKernelGGL(mlp_infer_kernel<4>, dim3((unsigned)grid), dim3(256), 0, stream, a, wt, params, pred);
+hipError_t launch_mlp_infer(const StepArgs &a, const uint16_t *wt, const float *params, float *pred, float *xraw,
+                            int64_t mp, hipStream_t stream) {
+  // one workgroup per CU (the kernel's LDS holds one), each looping over
+  // every gridDim.x-th block of 256 tiles
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  const int64_t blocks = (mp + infer::TILES - 1) / infer::TILES;
+  const int64_t grid = blocks < cus ? blocks : cus;
+  hipLaunchKernelGGL(mlp_infer_kernel<infer::WAVES>, dim3((unsigned)grid), dim3(64 * infer::WAVES), 0, stream, a, wt,
+                     params, pred, xraw);
   return hipGetLastError();
 }
 

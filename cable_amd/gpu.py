@@ -33,7 +33,7 @@ EXPORTS = [
     "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters", "cable_gpu_set_tile_order",
     "cable_gpu_run_capture", "cable_gpu_samples", "cable_gpu_copy_samples", "cable_gpu_set_surrogate",
     "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
-    "cable_gpu_download_forcing", "cable_gpu_last_error",
+    "cable_gpu_download_forcing", "cable_gpu_surrogate_eval", "cable_gpu_last_error",
 ]
 
 _lib = None
@@ -84,6 +84,7 @@ def lib():
         "cable_gpu_synth_forcing": ([vp, c_int, c_int, ctypes.c_uint64, c_double, c_double, c_double, vp, vp],
                                     c_int),
         "cable_gpu_download_forcing": ([vp, c_int, c_int, vp, c_int], c_int),
+        "cable_gpu_surrogate_eval": ([vp, c_int, vp, vp], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -273,6 +274,14 @@ class CableGPU:
     def run_hybrid(self, k0: int, nsteps: int, dels: float, max_passes=2, resid_max=0.5, use_surrogate=True):
         _check(lib().cable_gpu_run_hybrid(self.h, int(k0), int(nsteps), float(dels), int(max_passes),
                                           float(resid_max), 1 if use_surrogate else 0))
+
+    def surrogate_eval(self, k: int):
+        """(raw features (mp, 24), predictions (mp, 8)) of step k from the
+        current state, as the hybrid step's inference computes them."""
+        x = np.empty((self.mp, 24), dtype=np.float32)
+        p = np.empty((self.mp, 8), dtype=np.float32)
+        _check(lib().cable_gpu_surrogate_eval(self.h, int(k), _ptr(x), _ptr(p)))
+        return x, p
 
     def hybrid_counters(self):
         """(fallback tile-steps, tile-steps) of the last hybrid run."""

@@ -205,6 +205,12 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, in
                          double resid_max, int use_surrogate);
 /* fallback tile-steps and all tile-steps of the last hybrid run */
 int cable_gpu_hybrid_counters(cable_gpu_ctx *ctx, int64_t *fallback_tile_steps, int64_t *tile_steps);
+/* The surrogate's view of step k from the current state, as the hybrid step
+   computes it: x (mp x 24) the raw features, pred (mp x 8) the denormalised
+   predictions, host arrays in the caller's tile order.  Predictions of tiles
+   in 256-tile blocks without a leaf-bearing vegetation type are computed
+   too. */
+int cable_gpu_surrogate_eval(cable_gpu_ctx *ctx, int k, float *x, float *pred);
 
 const char *cable_gpu_last_error(void);
 

@@ -86,3 +86,32 @@ def test_network_warm_start_cuts_fallbacks_and_passes():
     print({f"surrogate={u} cap={c}": (round(v[0], 4), v[1]) for (u, c), v in res.items()}, flush=True)
     assert res[(True, 3)][0] < 0.5 * res[(False, 3)][0]
     assert res[(True, 20)][1] < res[(False, 20)][1]
+
+
+def test_inference_kernel_matches_oracle():
+    """The hybrid step's inference (features from the device state, then the
+    LDS-staged matrix-core network) against the CPU oracle of the network on
+    the same features, for every tile of a step."""
+    from cable_amd.surrogate import NIN, NOUT, norm_stats
+    from oracle import mlp_oracle as O
+    print("[hybrid] inference vs oracle", flush=True)
+    wl = Workload(3000, 2)                     # not a multiple of the 256-tile block
+    g = _ctx(wl)
+    p = O.init_params(11).astype(np.float32)
+    unit = np.concatenate([np.zeros(NIN), np.ones(NIN), np.zeros(NOUT), np.ones(NOUT)]).astype(np.float32)
+    g.set_surrogate(p, unit)
+    x0, _ = g.surrogate_eval(0)
+    assert np.isfinite(x0).all()
+    rng = np.random.default_rng(12)
+    norm = norm_stats(x0, rng.normal(3.0, 2.0, size=(x0.shape[0], NOUT)))
+    g.set_surrogate(p, norm)
+    x, pred = g.surrogate_eval(0)
+    g.close()
+    np.testing.assert_array_equal(x, x0)
+    mu_x, sd_x = norm[:NIN], norm[NIN:2 * NIN]
+    mu_y, sd_y = norm[2 * NIN:2 * NIN + NOUT], norm[2 * NIN + NOUT:]
+    ref, _ = O.forward(p.astype(np.float64), (x.astype(np.float64) - mu_x) / sd_x, bf16_ops=True)
+    got = (pred.astype(np.float64) - mu_y) / sd_y
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"inference max error / max |y| = {err:.2e}", flush=True)
+    assert err < 2e-3
