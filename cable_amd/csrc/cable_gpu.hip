@@ -143,9 +143,22 @@ hipError_t launch(const cable_gpu_ctx *c, const StepArgs &a, bool full) {
 }
 
 // host conversion between a host array and the device element type
+// the host array has the device's element type and tile order: copy as is
+bool direct_copy(const cable_gpu_ctx *c, const FieldMeta &f, int host_type) {
+  if (!c->order.empty()) return false;
+  if (f.prec == 'I') return host_type == CABLE_HOST_I32;
+  const size_t de = dev_elem(f, c->precision);
+  return (de == 8 && host_type == CABLE_HOST_F64) || (de == 4 && host_type == CABLE_HOST_F32);
+}
+
 int copy_to_device(cable_gpu_ctx *c, void *dst, const FieldMeta &f, const void *host, int host_type,
                    size_t count) {
   size_t de = dev_elem(f, c->precision);
+  if (direct_copy(c, f, host_type)) {
+    HIPCHECK(hipMemcpyAsync(dst, host, de * count, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CABLE_OK;
+  }
   std::vector<unsigned char> buf(de * count);
   const size_t mp = (size_t)c->mp;
   const int64_t *ord = c->order.empty() ? nullptr : c->order.data();
@@ -257,6 +270,11 @@ int cable_gpu_download(cable_gpu_ctx *c, int id, void *host, int host_type) {
   if (!c || !host || id < 0 || id >= CABLE_NFIELDS) return fail(CABLE_E_ARG, "cable_gpu_download: bad arguments");
   const FieldMeta &f = kFields[id];
   size_t count = (size_t)f.n * c->mp, de = dev_elem(f, c->precision);
+  if (direct_copy(c, f, host_type)) {
+    HIPCHECK(hipMemcpyAsync(host, c->dev[id], de * count, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CABLE_OK;
+  }
   std::vector<unsigned char> buf(de * count);
   HIPCHECK(hipMemcpyAsync(buf.data(), c->dev[id], buf.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));

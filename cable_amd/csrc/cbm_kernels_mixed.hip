@@ -31,13 +31,10 @@ hipError_t launch_mlp_infer(const StepArgs &a, const uint16_t *wt, const float *
                             int64_t mp, hipStream_t stream) {
   // one workgroup per CU (the kernel's LDS holds one), each looping over
   // every gridDim.x-th block of 256 tiles
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-      cus = 256;
-  }
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
   const int64_t blocks = (mp + infer::TILES - 1) / infer::TILES;
   const int64_t grid = blocks < cus ? blocks : cus;
   hipLaunchKernelGGL(mlp_infer_kernel<infer::WAVES>, dim3((unsigned)grid), dim3(64 * infer::WAVES), 0, stream, a, wt,
