@@ -108,6 +108,10 @@ struct cable_gpu_ctx {
   unsigned *qlen = nullptr;
   unsigned long long *qtotal = nullptr;
   int64_t last_fallbacks = 0, last_tile_steps = 0;
+  // ---- the driver's consistency check ----
+  bool cons_on = false;
+  double *cons_part = nullptr;      // per-workgroup sums of a step
+  double *cons_acc = nullptr;       // count_bal, new_sumbal, new_sumfpn, new_sumfe
 };
 
 namespace {
@@ -249,7 +253,7 @@ int cable_gpu_destroy(cable_gpu_ctx *c) {
   if (c->d_vtab) hipFree(c->d_vtab);
   if (c->d_stab) hipFree(c->d_stab);
   if (c->d_npass) hipFree(c->d_npass);
-  void *sur[] = {c->hyb, c->sx, c->sy, c->scount, c->sparams, c->snorm, c->pred, c->swt, c->queue, c->qlen,
+  void *sur[] = {c->cons_part, c->cons_acc, c->hyb, c->sx, c->sy, c->scount, c->sparams, c->snorm, c->pred, c->swt, c->queue, c->qlen,
                  c->qtotal};
   for (void *p : sur)
     if (p) hipFree(p);
@@ -425,6 +429,61 @@ int cable_gpu_set_forcing(cable_gpu_ctx *c, int nsteps, const void *fsd1, const 
 }  // extern "C"
 
 namespace {
+// ---- the offline driver's consistency check -------------------------------
+constexpr int CONS_THREADS = 256;
+
+// in-order tree over the threads of one workgroup (CONS_THREADS, a power of 2)
+__device__ inline void block_sum(double (*sh)[4], int nq) {
+  for (int w = CONS_THREADS / 2; w > 0; w >>= 1) {
+    __syncthreads();
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < nq; q++) sh[threadIdx.x][q] += sh[threadIdx.x + w][q];
+  }
+  __syncthreads();
+}
+
+// one step's tile sums from the step kernel's per-workgroup sums, added to the
+// running sums as src/offline/cable_serial.F90:748-752 does:
+// new_sumbal += SUM(wbal)/mp + SUM(ebal)/mp, new_sumfpn += SUM(fpn)/mp,
+// new_sumfe += SUM(fe)/mp, count_bal += 1
+__global__ void __launch_bounds__(CONS_THREADS) cons_reduce_kernel(const double *part, int64_t nb, double mp,
+                                                                  double *acc) {
+  __shared__ double sh[CONS_THREADS][4];
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = threadIdx.x; i < nb; i += CONS_THREADS)
+    for (int q = 0; q < 4; q++) s[q] += part[i * 4 + q];
+  for (int q = 0; q < 4; q++) sh[threadIdx.x][q] = s[q];
+  block_sum(sh, 4);
+  if (threadIdx.x == 0) {
+    acc[0] = acc[0] + 1.0;
+    acc[1] = acc[1] + sh[0][0] / mp + sh[0][1] / mp;
+    acc[2] = acc[2] + sh[0][2] / mp;
+    acc[3] = acc[3] + sh[0][3] / mp;
+  }
+}
+
+// SUM(x) over n doubles, one workgroup, fixed order
+__global__ void __launch_bounds__(CONS_THREADS) sum_kernel(const double *x, int64_t n, double *out) {
+  __shared__ double sh[CONS_THREADS][4];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += CONS_THREADS) s += x[i];
+  sh[threadIdx.x][0] = s;
+  block_sum(sh, 1);
+  if (threadIdx.x == 0) *out = sh[0][0];
+}
+
+// workgroups of a step launch (launch_cbm: 64 threads for FP64, 256 for FP32)
+int64_t step_workgroups(const cable_gpu_ctx *c) {
+  const int64_t block = c->precision == CABLE_PREC_MIXED ? 256 : 64;
+  return (c->mp + block - 1) / block;
+}
+
+hipError_t launch_cons_reduce(cable_gpu_ctx *c) {
+  hipLaunchKernelGGL(cons_reduce_kernel, dim3(1), dim3(CONS_THREADS), 0, c->stream, c->cons_part,
+                     step_workgroups(c), (double)c->mp, c->cons_acc);
+  return hipGetLastError();
+}
+
 // SplitMix64's increment and finaliser (Steele, Lea and Flood, "Fast
 // splittable pseudorandom number generators", OOPSLA 2014): mix64(0) is the
 // first output of SplitMix64 seeded with 0, 0xE220A8397B1DCDAF
@@ -595,6 +654,43 @@ int cable_gpu_download_forcing(cable_gpu_ctx *c, int k, int which, void *host, i
   return CABLE_OK;
 }
 
+int cable_gpu_consistency_check(cable_gpu_ctx *c, int enable) {
+  if (!c) return fail(CABLE_E_ARG, "cable_gpu_consistency_check: null context");
+  if (enable) {
+    if (!c->cons_part) HIPCHECK(hipMalloc(&c->cons_part, sizeof(double) * 4 * (size_t)step_workgroups(c)));
+    if (!c->cons_acc) HIPCHECK(hipMalloc(&c->cons_acc, sizeof(double) * 4));
+    HIPCHECK(hipMemsetAsync(c->cons_acc, 0, sizeof(double) * 4, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
+  c->cons_on = enable != 0;
+  return CABLE_OK;
+}
+
+int cable_gpu_consistency_values(cable_gpu_ctx *c, double *out) {
+  if (!c || !out) return fail(CABLE_E_ARG, "cable_gpu_consistency_values: null argument");
+  if (!c->cons_acc) return fail(CABLE_E_STATE, "cable_gpu_consistency_values: the check was never enabled");
+  double *tot = nullptr;
+  HIPCHECK(hipMalloc(&tot, sizeof(double) * 2));
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(CONS_THREADS), 0, c->stream, (const double *)c->dev[CABLE_F_bal_ebal_tot],
+                     c->mp, tot);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(CONS_THREADS), 0, c->stream, (const double *)c->dev[CABLE_F_bal_wbal_tot],
+                     c->mp, tot + 1);
+  HIPCHECK(hipGetLastError());
+  double acc[4], t2[2];
+  HIPCHECK(hipMemcpyAsync(acc, c->cons_acc, sizeof acc, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(t2, tot, sizeof t2, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipFree(tot));
+  const double n = acc[0], mp = (double)c->mp;
+  out[0] = n;
+  out[1] = acc[1];
+  out[2] = n > 0 ? acc[3] / n : 0.0;
+  out[3] = n > 0 ? acc[2] / n : 0.0;
+  out[4] = n > 0 ? t2[0] / mp / n : 0.0;
+  out[5] = n > 0 ? t2[1] / mp / n : 0.0;
+  return CABLE_OK;
+}
+
 int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
   if (!c) return fail(CABLE_E_ARG, "cable_gpu_run: null context");
   if (!c->have_tables || c->nsteps_forc == 0)
@@ -607,7 +703,9 @@ int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   for (int k = k0; k < k0 + nsteps; k++) {
     StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)(k - c->forc_k0) * c->mp);
+    if (c->cons_on) a.cons_part = c->cons_part;
     hipError_t e = launch(c, a, false);
+    if (e == hipSuccess && c->cons_on) e = launch_cons_reduce(c);
     if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run launch: %s", hipGetErrorString(e));
     c->first_call = false;
   }

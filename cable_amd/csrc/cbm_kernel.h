@@ -122,7 +122,10 @@ __device__ inline unsigned wave_append32(bool take, unsigned *count) {
   return base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
-template <class R, bool FULL, bool TABLES, int MODE = MODE_PLAIN>
+// CONS: also form the workgroup's sums for the driver's consistency check
+// (a separate instantiation: the check's code costs the plain step 2-3% even
+// when it is switched off at run time)
+template <class R, bool FULL, bool TABLES, int MODE = MODE_PLAIN, bool CONS = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 2 : 1)))
 cbm_kernel(StepArgs a) {
   __shared__ double lds_v[TABLES ? NVEGCLASS * VC_N : 1];
@@ -146,6 +149,7 @@ cbm_kernel(StepArgs a) {
   }
   int passes = 0;
   bool fallback = false, emit = false;
+  double cons[CONS ? 4 : 1] = {0.0};   // this tile's wbal, ebal, fpn, fe (CONS)
   float feat[MODE == MODE_CAPTURE ? mlp::NIN : 1];
   float tgt[MODE == MODE_CAPTURE ? mlp::NOUT : 1];
   if (live) {
@@ -261,6 +265,12 @@ cbm_kernel(StepArgs a) {
     }
     S.run(late);
     passes = S.npass;
+    if constexpr (CONS) {
+      cons[0] = (double)T.bal_wbal[0];
+      cons[1] = (double)T.bal_ebal[0];
+      cons[2] = (double)T.canopy_fpn[0];
+      cons[3] = (double)T.canopy_fe[0];
+    }
     if constexpr (MODE == MODE_HYBRID) fallback = S.vegetated && S.resid > a.resid_max;
     if constexpr (MODE == MODE_CAPTURE) {
       emit = S.vegetated;
@@ -317,6 +327,24 @@ cbm_kernel(StepArgs a) {
       STD_(canopy_fes, 1) STS_(canopy_fgpp, 1) STS_(ssnow_runoff, 1) STS_(rad_trad, 1)
 #undef STS_
 #undef STD_
+    }
+  }
+  if constexpr (CONS) {
+    // workgroup sums in a fixed order: a butterfly within each wave, then
+    // the waves in order
+    __shared__ double wsum[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      for (int o = 32; o > 0; o >>= 1) cons[q] += __shfl_xor(cons[q], o, 64);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int q = 0; q < 4; q++) wsum[w][q] = cons[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double s = 0.0;
+      for (int i = 0; i < nw; i++) s += wsum[i][threadIdx.x];
+      a.cons_part[(int64_t)blockIdx.x * 4 + threadIdx.x] = s;
     }
   }
   if (a.npass) {
@@ -516,14 +544,14 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
 
 
 // launch one step over mp tiles on `stream` (256 threads per workgroup)
-template <class R, bool FULL, bool TABLES, int MODE = MODE_PLAIN>
+template <class R, bool FULL, bool TABLES, int MODE = MODE_PLAIN, bool CONS = false>
 hipError_t launch_cbm(const StepArgs &a, int64_t mp, hipStream_t stream) {
   // FP64 runs one wave per SIMD: one-wave workgroups let the dispatcher fill
   // SIMDs wave by wave (measured 2-4% faster than four-wave workgroups); the
   // FP32 kernels keep four waves per workgroup, two resident per SIMD
   const int block = sizeof(R) == 8 ? 64 : 256;
   const int64_t grid = (mp + block - 1) / block;
-  hipLaunchKernelGGL((cbm_kernel<R, FULL, TABLES, MODE>), dim3((unsigned)grid), dim3(block), 0, stream, a);
+  hipLaunchKernelGGL((cbm_kernel<R, FULL, TABLES, MODE, CONS>), dim3((unsigned)grid), dim3(block), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -6,7 +6,9 @@
 namespace cbm {
 
 hipError_t launch_fp64(bool full, const StepArgs &a, int64_t mp, hipStream_t stream) {
-  return full ? launch_cbm<double, true, false>(a, mp, stream) : launch_cbm<double, false, true>(a, mp, stream);
+  if (full) return launch_cbm<double, true, false>(a, mp, stream);
+  return a.cons_part ? launch_cbm<double, false, true, MODE_PLAIN, true>(a, mp, stream)
+                     : launch_cbm<double, false, true>(a, mp, stream);
 }
 
 }  // namespace cbm

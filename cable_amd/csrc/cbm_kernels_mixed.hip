@@ -7,7 +7,9 @@
 namespace cbm {
 
 hipError_t launch_mixed(bool full, const StepArgs &a, int64_t mp, hipStream_t stream) {
-  return full ? launch_cbm<float, true, false>(a, mp, stream) : launch_cbm<float, false, true>(a, mp, stream);
+  if (full) return launch_cbm<float, true, false>(a, mp, stream);
+  return a.cons_part ? launch_cbm<float, false, true, MODE_PLAIN, true>(a, mp, stream)
+                     : launch_cbm<float, false, true>(a, mp, stream);
 }
 
 hipError_t launch_mixed_mode(int mode, const StepArgs &a, int64_t mp, hipStream_t stream) {

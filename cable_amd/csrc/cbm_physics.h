@@ -31,9 +31,13 @@
 #include "../../include/cable_fields.h"
 
 // device code; the same functions also compile for the host so the test
-// suite can run the kernel's physics on the CPU
+// suite can run the kernel's physics on the CPU.  Everything is inlined into
+// the step kernel: the tile's working set only stays in registers when no
+// routine is left as a call (with several kernel instantiations in one
+// translation unit the compiler would otherwise outline shared routines and
+// pass the working set through scratch memory)
 #ifndef CBM_HD
-#define CBM_HD __host__ __device__
+#define CBM_HD __host__ __device__ __forceinline__
 #endif
 
 namespace cbm {
@@ -133,6 +137,9 @@ struct StepArgs {
                          // precip_sn from tk, ca and doy scalars
   double ca, doy;
   unsigned long long *npass;  // optional: sum of dryLeaf passes (work counter)
+  double *cons_part;     // optional: per-workgroup sums of bal%wbal, bal%ebal,
+                         // canopy%fpn, canopy%fe ([gridDim.x][4]) for the
+                         // driver's consistency check
   // ---- surrogate (mixed path only) ----
   float *hyb;                // [3][mp]: previous step's leaf and canopy-air
                              // temperature, canopy-air humidity (or null)

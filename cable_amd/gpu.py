@@ -33,7 +33,8 @@ EXPORTS = [
     "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters", "cable_gpu_set_tile_order",
     "cable_gpu_run_capture", "cable_gpu_samples", "cable_gpu_copy_samples", "cable_gpu_set_surrogate",
     "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
-    "cable_gpu_download_forcing", "cable_gpu_surrogate_eval", "cable_gpu_last_error",
+    "cable_gpu_download_forcing", "cable_gpu_surrogate_eval", "cable_gpu_consistency_check",
+    "cable_gpu_consistency_values", "cable_gpu_last_error",
 ]
 
 _lib = None
@@ -85,6 +86,8 @@ def lib():
                                     c_int),
         "cable_gpu_download_forcing": ([vp, c_int, c_int, vp, c_int], c_int),
         "cable_gpu_surrogate_eval": ([vp, c_int, vp, vp], c_int),
+        "cable_gpu_consistency_check": ([vp, c_int], c_int),
+        "cable_gpu_consistency_values": ([vp, vp], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -296,6 +299,17 @@ class CableGPU:
 
     def sync(self):
         _check(lib().cable_gpu_sync(self.h))
+
+    def consistency_check(self, enable=True):
+        """Start (zeroing the sums) or stop the driver's consistency sums."""
+        _check(lib().cable_gpu_consistency_check(self.h, 1 if enable else 0))
+
+    def consistency_values(self):
+        """{count_bal, new_sumbal, sum_fe, sum_fpn, ebal_tot, wbal_tot} as the
+        offline driver forms them (cable_serial.F90:748-771)."""
+        out = np.zeros(6)
+        _check(lib().cable_gpu_consistency_values(self.h, _ptr(out)))
+        return dict(zip(["count_bal", "new_sumbal", "sum_fe", "sum_fpn", "ebal_tot", "wbal_tot"], out.tolist()))
 
 
 def class_order(iveg, isoilm=None, vlai=None):

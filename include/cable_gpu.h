@@ -158,6 +158,20 @@ int cable_gpu_download_forcing(cable_gpu_ctx *ctx, int k, int which, void *host,
 int cable_gpu_run(cable_gpu_ctx *ctx, int k0, int nsteps, double dels);
 int cable_gpu_sync(cable_gpu_ctx *ctx);
 
+/* The offline driver's consistency check (src/offline/cable_serial.F90:
+   748-799, run when cable_user%consistency_check is set).  enable = 1 zeroes
+   the running sums; each later step of cable_gpu_run then adds, over all
+   tiles, new_sumbal += SUM(bal%wbal)/mp + SUM(bal%ebal)/mp,
+   new_sumfpn += SUM(canopy%fpn)/mp, new_sumfe += SUM(canopy%fe)/mp and
+   count_bal += 1, in FP64 and in a fixed order on the device.  enable = 0
+   stops adding. */
+int cable_gpu_consistency_check(cable_gpu_ctx *ctx, int enable);
+/* out[6]: count_bal, new_sumbal (the value compare_consistency_check_values
+   checks), new_sumfe/count_bal, new_sumfpn/count_bal,
+   SUM(bal%ebal_tot)/mp/count_bal and SUM(bal%wbal_tot)/mp/count_bal (what the
+   driver prints at the last step, cable_serial.F90:756-771). */
+int cable_gpu_consistency_values(cable_gpu_ctx *ctx, double *out);
+
 /* Device time of the last cable_gpu_run, in milliseconds, measured with HIP
    events on the context stream around its kernels. */
 int cable_gpu_last_run_ms(cable_gpu_ctx *ctx, double *ms);

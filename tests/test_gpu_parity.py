@@ -179,3 +179,41 @@ def test_tile_order_is_invisible(precision):
     c, _ = _trajectories(wl, precision, names, full=True, order=gpu.class_order(wl.iveg))
     d, _ = _trajectories(wl, precision, names, full=True)
     assert all(np.array_equal(c[k], d[k]) for k in names)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_consistency_check_sums(precision):
+    """The driver's consistency check summed on the device (resident run)
+    against the same sums over the per-tile balances of the full-diagnostic
+    step, formed as src/offline/cable_serial.F90:748-771 forms them."""
+    from cable_amd import gpu
+    print(f"[parity] consistency check precision={precision}", flush=True)
+    wl = snowy(Workload(1001, 12))
+    names = ["bal%wbal", "bal%ebal", "canopy%fpn", "canopy%fe", "bal%ebal_tot", "bal%wbal_tot"]
+    ref, _ = _trajectories(wl, precision, names, full=True)
+    mp, n = wl.n, wl.nsteps
+    want = {"count_bal": float(n),
+            "new_sumbal": float(sum(ref["bal%wbal"][k].sum() / mp + ref["bal%ebal"][k].sum() / mp for k in range(n))),
+            "sum_fe": float(sum(ref["canopy%fe"][k].sum() / mp for k in range(n)) / n),
+            "sum_fpn": float(sum(ref["canopy%fpn"][k].sum() / mp for k in range(n)) / n),
+            "ebal_tot": float(ref["bal%ebal_tot"][-1].sum() / mp / n),
+            "wbal_tot": float(ref["bal%wbal_tot"][-1].sum() / mp / n)}
+    g = gpu.from_workload(wl, precision=precision, full=False, order=gpu.class_order(wl.iveg, wl.isoilm, wl.vlai))
+    g.consistency_check(True)
+    g.run(0, 6, wl.dels)
+    g.run(6, 6, wl.dels)
+    got = g.consistency_values()
+    g.consistency_check(False)
+    g.run(0, 1, wl.dels)                     # stopped: the running sums stay
+    after = g.consistency_values()
+    assert all(after[k] == got[k] for k in ("count_bal", "new_sumbal", "sum_fe", "sum_fpn"))
+    g.close()
+    print({k: (got[k], want[k]) for k in want}, flush=True)
+    assert got["count_bal"] == n
+    for k in want:
+        if precision == 0:
+            # the same per-tile values, summed in another order
+            assert abs(got[k] - want[k]) <= 1e-11 * max(1.0, abs(want[k])), k
+        else:
+            # the resident mixed run is held to the flux tolerance of the full step
+            assert abs(got[k] - want[k]) <= 0.05 * max(1.0, abs(want[k])), k
