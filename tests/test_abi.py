@@ -85,3 +85,20 @@ def test_errors_without_a_context(L):
     assert L.cable_gpu_step(None, 1, 1800.0) == 1
     assert b"null" in L.cable_gpu_last_error()
     assert L.cable_gpu_run(None, 0, 1, 1800.0) == 1
+
+
+def test_new_entry_points_reject_bad_arguments(L):
+    """Argument checks of the forcing generator, the consistency check and
+    the surrogate evaluation, before any device work."""
+    import ctypes
+    d = ctypes.c_double
+    assert L.cable_gpu_synth_forcing(None, 0, 1, 1, d(1800.0), d(172.0), d(0.0), None, None) == 1
+    assert L.cable_gpu_download_forcing(None, 0, 0, None, 0) == 1
+    assert L.cable_gpu_consistency_check(None, 1) == 1
+    assert L.cable_gpu_consistency_values(None, None) == 1
+    assert L.cable_gpu_surrogate_eval(None, 0, None, None) == 1
+    assert L.cable_gpu_run_hybrid(None, 0, 1, d(1800.0), 4, d(0.5), 1) == 1
+    assert L.cable_gpu_create(ctypes.byref(ctypes.c_void_p()), 0, 0) == 1
+    assert b"bad arguments" in L.cable_gpu_last_error()
+    assert L.cable_gpu_create(ctypes.byref(ctypes.c_void_p()), 16, 7) == 1
+    assert b"precision" in L.cable_gpu_last_error()

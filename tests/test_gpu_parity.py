@@ -56,7 +56,7 @@ def _upload_forcing(g, forcing, s):
             g.upload(k, v[s])
 
 
-@pytest.mark.parametrize("n,snow", [(153, False), (612, True)])
+@pytest.mark.parametrize("n,snow", [(1, False), (153, False), (612, True)])
 def test_each_step_fp64_matches_oracle(n, snow):
     """Before each checked step the GPU receives the oracle's complete field
     set, so every step is compared from identical state."""
