@@ -299,6 +299,109 @@ int cable_gpu_download(cable_gpu_ctx *c, int id, void *host, int host_type) {
   return CABLE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// state save and load (restart)
+
+}  // extern "C"
+
+namespace {
+// header of a saved state: magic, layout version, mp, precision, field count
+// and the field table's checksum, then each saved field's (id, n, element
+// size), then the fields' device-typed values in the caller's tile order
+constexpr uint32_t STATE_MAGIC = 0x534C4243u;  // "CBLS"
+constexpr uint32_t STATE_VERSION = 1;
+struct StateHeader {
+  uint32_t magic, version;
+  int64_t mp;
+  int32_t precision, nsaved;
+  uint64_t table_sum;
+};
+bool saved_field(int id) { return kFields[id].role == ROLE_STATE || kFields[id].role == ROLE_DIAG; }
+uint64_t table_checksum() {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over names, precisions and counts
+  for (int id = 0; id < CABLE_NFIELDS; id++) {
+    for (const char *q = kFields[id].name; *q; q++) h = (h ^ (uint8_t)*q) * 1099511628211ull;
+    h = (h ^ (uint8_t)kFields[id].prec) * 1099511628211ull;
+    h = (h ^ (uint64_t)kFields[id].n) * 1099511628211ull;
+  }
+  return h;
+}
+int64_t state_bytes(const cable_gpu_ctx *c) {
+  int64_t b = sizeof(StateHeader);
+  for (int id = 0; id < CABLE_NFIELDS; id++)
+    if (saved_field(id)) b += 3 * sizeof(int32_t) + (int64_t)field_bytes(c, id);
+  return b;
+}
+int host_type_of(const FieldMeta &f, int precision) {
+  if (f.prec == 'I') return CABLE_HOST_I32;
+  return dev_elem(f, precision) == 8 ? CABLE_HOST_F64 : CABLE_HOST_F32;
+}
+}  // namespace
+
+extern "C" {
+
+int64_t cable_gpu_state_bytes(cable_gpu_ctx *c) {
+  if (!c) return -(int64_t)fail(CABLE_E_ARG, "cable_gpu_state_bytes: null context");
+  return state_bytes(c);
+}
+
+int cable_gpu_save_state(cable_gpu_ctx *c, void *buf, int64_t bytes) {
+  if (!c || !buf) return fail(CABLE_E_ARG, "cable_gpu_save_state: null argument");
+  if (bytes < state_bytes(c))
+    return fail(CABLE_E_ARG, "cable_gpu_save_state: buffer of %lld bytes, %lld needed", (long long)bytes,
+                (long long)state_bytes(c));
+  unsigned char *p = (unsigned char *)buf;
+  StateHeader h = {STATE_MAGIC, STATE_VERSION, c->mp, c->precision, 0, table_checksum()};
+  for (int id = 0; id < CABLE_NFIELDS; id++) h.nsaved += saved_field(id) ? 1 : 0;
+  memcpy(p, &h, sizeof h);
+  p += sizeof h;
+  for (int id = 0; id < CABLE_NFIELDS; id++) {
+    if (!saved_field(id)) continue;
+    const FieldMeta &f = kFields[id];
+    const int32_t rec[3] = {id, f.n, (int32_t)dev_elem(f, c->precision)};
+    memcpy(p, rec, sizeof rec);
+    p += sizeof rec;
+    int r = cable_gpu_download(c, id, p, host_type_of(f, c->precision));
+    if (r) return r;
+    p += field_bytes(c, id);
+  }
+  return CABLE_OK;
+}
+
+int cable_gpu_load_state(cable_gpu_ctx *c, const void *buf, int64_t bytes, int continue_run) {
+  if (!c || !buf) return fail(CABLE_E_ARG, "cable_gpu_load_state: null argument");
+  if (bytes < (int64_t)sizeof(StateHeader)) return fail(CABLE_E_ARG, "cable_gpu_load_state: truncated state");
+  const unsigned char *p = (const unsigned char *)buf;
+  StateHeader h;
+  memcpy(&h, p, sizeof h);
+  if (h.magic != STATE_MAGIC || h.version != STATE_VERSION)
+    return fail(CABLE_E_ARG, "cable_gpu_load_state: not a saved state of this library");
+  if (h.table_sum != table_checksum())
+    return fail(CABLE_E_ARG, "cable_gpu_load_state: saved with a different field table");
+  if (h.mp != c->mp || h.precision != c->precision)
+    return fail(CABLE_E_ARG, "cable_gpu_load_state: saved for mp=%lld precision %d, context has mp=%lld precision %d",
+                (long long)h.mp, h.precision, (long long)c->mp, c->precision);
+  if (bytes < state_bytes(c)) return fail(CABLE_E_ARG, "cable_gpu_load_state: truncated state");
+  p += sizeof h;
+  for (int i = 0; i < h.nsaved; i++) {
+    int32_t rec[3];
+    memcpy(rec, p, sizeof rec);
+    p += sizeof rec;
+    const int id = rec[0];
+    if (id < 0 || id >= CABLE_NFIELDS || !saved_field(id) || rec[1] != kFields[id].n ||
+        rec[2] != (int32_t)dev_elem(kFields[id], c->precision))
+      return fail(CABLE_E_ARG, "cable_gpu_load_state: bad field record %d", i);
+    int r = cable_gpu_upload(c, id, p, host_type_of(kFields[id], c->precision));
+    if (r) return r;
+    p += field_bytes(c, id);
+  }
+  // an exact continuation keeps the run's first-call state; otherwise the next
+  // step is the first of a new run, as a reference run started from a restart
+  c->first_call = continue_run == 0;
+  c->hyb_valid = false;
+  return CABLE_OK;
+}
+
 int cable_gpu_set_tile_order(cable_gpu_ctx *c, const int64_t *order) {
   if (!c || !order) return fail(CABLE_E_ARG, "cable_gpu_set_tile_order: null argument");
   if (c->nsteps_forc)

@@ -34,7 +34,8 @@ EXPORTS = [
     "cable_gpu_run_capture", "cable_gpu_samples", "cable_gpu_copy_samples", "cable_gpu_set_surrogate",
     "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
     "cable_gpu_download_forcing", "cable_gpu_surrogate_eval", "cable_gpu_consistency_check",
-    "cable_gpu_consistency_values", "cable_gpu_last_error",
+    "cable_gpu_consistency_values", "cable_gpu_state_bytes", "cable_gpu_save_state", "cable_gpu_load_state",
+    "cable_gpu_last_error",
 ]
 
 _lib = None
@@ -87,6 +88,9 @@ def lib():
         "cable_gpu_download_forcing": ([vp, c_int, c_int, vp, c_int], c_int),
         "cable_gpu_surrogate_eval": ([vp, c_int, vp, vp], c_int),
         "cable_gpu_consistency_check": ([vp, c_int], c_int),
+        "cable_gpu_state_bytes": ([vp], i64),
+        "cable_gpu_save_state": ([vp, vp, i64], c_int),
+        "cable_gpu_load_state": ([vp, vp, i64, c_int], c_int),
         "cable_gpu_consistency_values": ([vp, vp], c_int),
     }
     for name, (args, res) in sig.items():
@@ -299,6 +303,21 @@ class CableGPU:
 
     def sync(self):
         _check(lib().cable_gpu_sync(self.h))
+
+    def save_state(self) -> bytes:
+        """The device state (every STATE and DIAG field) as one buffer."""
+        n = lib().cable_gpu_state_bytes(self.h)
+        if n < 0:
+            _check(-n)
+        buf = ctypes.create_string_buffer(n)
+        _check(lib().cable_gpu_save_state(self.h, buf, n))
+        return buf.raw
+
+    def load_state(self, state: bytes, continue_run=True):
+        """Load a saved state; continue_run: the next step continues the
+        saved run exactly (else it is the first step of a new run)."""
+        buf = ctypes.create_string_buffer(state, len(state))
+        _check(lib().cable_gpu_load_state(self.h, buf, len(state), 1 if continue_run else 0))
 
     def consistency_check(self, enable=True):
         """Start (zeroing the sums) or stop the driver's consistency sums."""

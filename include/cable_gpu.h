@@ -85,6 +85,25 @@ int cable_gpu_download(cable_gpu_ctx *ctx, int field_id, void *host, int host_ty
    in the reference, whose tiles are independent in any order.) */
 int cable_gpu_set_tile_order(cable_gpu_ctx *ctx, const int64_t *order);
 
+/* Saved state (restart).  cable_gpu_save_state packs every STATE and DIAG
+   field of include/cable_fields.h -- the prognostic state, the hidden state
+   the reference carries between steps (wbtot1/wbtot2, gammzz(:,1), owetfac,
+   the driver's owb) and the diagnostics that keep values on tiles a step
+   does not touch -- in the device's element types and the caller's tile
+   order, behind a header that records mp, the precision and the field
+   table.  cable_gpu_state_bytes gives the size (negative: error).
+   cable_gpu_load_state checks the header and uploads the fields; with
+   continue_run = 1 the next step continues the saved run exactly, with 0 it
+   is the first step of a new run, as the reference started from a restart
+   (first-call gammzz(:,1), src/science/soilsnow/cbl_soilsnow_main.F90:60-62).
+   Parameters and forcing are not part of the state.  Replaces, for this
+   path, the reference's restart output and input
+   (src/util/output/cable_output.F90:373-376, load_parameters in
+   src/offline/cable_parameters.F90). */
+int64_t cable_gpu_state_bytes(cable_gpu_ctx *ctx);
+int cable_gpu_save_state(cable_gpu_ctx *ctx, void *buf, int64_t bytes);
+int cable_gpu_load_state(cable_gpu_ctx *ctx, const void *buf, int64_t bytes, int continue_run);
+
 /* Grid-wide parameters: soil%zse(6), soil%zshh(7), bgc%ratecp(3),
    bgc%ratecs(2) (src/offline/cable_define_types.F90:119-213, 706-717). */
 int cable_gpu_set_globals(cable_gpu_ctx *ctx, const double *zse, const double *zshh,

@@ -217,3 +217,36 @@ def test_consistency_check_sums(precision):
         else:
             # the resident mixed run is held to the flux tolerance of the full step
             assert abs(got[k] - want[k]) <= 0.05 * max(1.0, abs(want[k])), k
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_saved_state_continues_the_run_exactly(precision):
+    """Save after 12 steps, load into a fresh context (another tile order),
+    run 12 more: the same bits as 24 steps without a break."""
+    from cable_amd import gpu
+    print(f"[parity] save/load state precision={precision}", flush=True)
+    wl = snowy(Workload(1001, 24))
+    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai)
+    names = HEADLINE + ["ssnow%tgg", "ssnow%wb", "ssnow%snowd", "canopy%cansto", "bal%wbal_tot"]
+    g = gpu.from_workload(wl, precision=precision, full=False, order=order)
+    g.run(0, 24, wl.dels)
+    want = {k: g.download(k) for k in names}
+    g.close()
+    g = gpu.from_workload(wl, precision=precision, full=False, order=order)
+    g.run(0, 12, wl.dels)
+    state = g.save_state()
+    g.close()
+    h = gpu.from_workload(wl, precision=precision, full=False)       # caller order
+    with pytest.raises(gpu.CableError):
+        h.load_state(state[:100])
+    h.load_state(state)
+    h.run(12, 12, wl.dels)
+    got = {k: h.download(k) for k in names}
+    h.close()
+    diff = [k for k in names if not np.array_equal(got[k], want[k])]
+    assert not diff, diff
+    # a state saved for another size is refused
+    s = gpu.from_workload(Workload(64, 1), precision=precision, full=False)
+    with pytest.raises(gpu.CableError):
+        s.load_state(state)
+    s.close()
