@@ -248,6 +248,20 @@ cbm_kernel(StepArgs a) {
         LDL_(DT, ssnow_tgg, 1, 6) LDL_(DT, ssnow_tggsn, 1, 3) LDL_(double, ssnow_wb, 1, 6)
         LDL_(double, ssnow_wbice, 1, 6)
 #undef LDL_
+        // values the canopy reads only before its stability iterations and
+        // soil, snow and the balances read again: the shortwave forcing, the
+        // precipitation and its snow part, and the step's starting surface
+        // temperature (otss, which the step sets from the stored tss).  FP64
+        // only: the FP32 kernel's register allocation spills more with it
+        if constexpr (sizeof(R) == 8) {
+          const int64_t i = t + a.foff;
+          const R fsd = ld<DT>(F[CABLE_F_met_fsd], i), pr = ld<DT>(F[CABLE_F_met_precip], i);
+          T.met_fsd[0] = fsd;
+          T.met_fsd[1] = fsd;
+          T.met_precip[0] = pr;
+          T.met_precip_sn[0] = (T.met_tk[0] <= R(273.16)) ? pr : R(0.0);
+          T.ssnow_otss[0] = ld<DT>(F[CABLE_F_ssnow_tss], t);
+        }
         LDS_(ssnow_smass, 3) LDS_(ssnow_ssdn, 3) LDS_(bgc_cplant, 3) LDS_(bgc_csoil, 2)
         LDD_(drv_owb, 1) LDD_(bal_ebal_tot, 1) LDD_(bal_wbal_tot, 1) LDD_(bal_precip_tot, 1)
         LDD_(bal_rnoff_tot, 1) LDD_(bal_evap_tot, 1) LDD_(bal_radbalsum, 1)
