@@ -48,15 +48,19 @@ def hipcc():
 def build_gpu(verbose=False):
     objdir = os.path.join(ROOT, "cable_amd", "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    import concurrent.futures as cf
+    objs, cmds = [], []
     for src, extra in TU_FLAGS.items():
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [hipcc()] + HIPCC_FLAGS + extra + ["-c", "-o", obj, os.path.join(CSRC, src)]
-        r = subprocess.run(cmd, capture_output=True, text=True)
+        cmds.append((src, [hipcc()] + HIPCC_FLAGS + extra + ["-c", "-o", obj, os.path.join(CSRC, src)]))
+        objs.append(obj)
+    # the translation units compile independently: one process each
+    with cf.ThreadPoolExecutor(max(1, min(len(cmds), os.cpu_count() or 1))) as ex:
+        results = list(ex.map(lambda c: (c[0], subprocess.run(c[1], capture_output=True, text=True)), cmds))
+    for src, r in results:
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"hipcc failed for {src}")
-        objs.append(obj)
     tmp = LIB + ".tmp"
     r = subprocess.run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs,
                        capture_output=True, text=True)
