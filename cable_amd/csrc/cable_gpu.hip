@@ -82,6 +82,7 @@ struct cable_gpu_ctx {
   std::vector<void *> forc;         // resident forcing arrays (8)
   int nsteps_forc = 0;
   int forc_k0 = 0;                  // step of the first resident forcing slot
+  std::vector<double> forc_doy;     // per-step day of year of the window (set_met), or empty
   double ca = 400e-6, doy = 1.0;
   double zse[6] = {0}, zshh[7] = {0}, ratecp[3] = {0}, ratecs[2] = {0};
   double *d_vtab = nullptr, *d_stab = nullptr;
@@ -136,6 +137,11 @@ StepArgs make_args(const cable_gpu_ctx *c, void **tab, int ktau, double dels, in
   a.stab = c->d_stab;
   a.ca = c->ca;
   a.doy = c->doy;
+  if (tab == c->d_tab_run && !c->forc_doy.empty()) {
+    // the window's day of year at this step (cable_gpu_set_met)
+    const int64_t slot = (int64_t)ktau - 1 - c->forc_k0;
+    if (slot >= 0 && slot < (int64_t)c->forc_doy.size()) a.doy = c->forc_doy[(size_t)slot];
+  }
   a.npass = c->d_npass;
   a.driver = 1;
   return a;
@@ -521,6 +527,7 @@ int cable_gpu_set_forcing(cable_gpu_ctx *c, int nsteps, const void *fsd1, const 
   HIPCHECK(hipStreamSynchronize(c->stream));
   c->nsteps_forc = nsteps;
   c->forc_k0 = 0;
+  c->forc_doy.clear();
   c->ca = ca;
   c->doy = doy;
   return CABLE_OK;
@@ -585,6 +592,144 @@ hipError_t launch_cons_reduce(cable_gpu_ctx *c) {
   hipLaunchKernelGGL(cons_reduce_kernel, dim3(1), dim3(CONS_THREADS), 0, c->stream, c->cons_part,
                      step_workgroups(c), (double)c->mp, c->cons_acc);
   return hipGetLastError();
+}
+
+// ---- met forcing in the met file's variables and units ---------------------
+// the reader's saturation vapour pressure and specific humidity from
+// relative humidity, src/offline/cable_checks.F90:633-696, in precision R
+template <class R>
+__device__ inline R met_svp(R tk) {
+  const R temp = tk - R(273.15);
+  if (temp < R(-20.0)) {
+    const R toot = R(273.15) / tk, toto = R(1.) / toot;
+    const R eilog = R(-9.09718) * (toot - R(1)) - R(3.56654) * (log(toot) / log(R(10.0))) +
+                    R(0.876793) * (R(1) - toto) + (log(R(6.1071)) / log(R(10.0)));
+    return pow(R(10.0), eilog);
+  }
+  const R tsot = R(373.15) / tk;
+  const R ewlog = R(-7.90298) * (tsot - R(1)) + R(5.02808) * (log(tsot) / log(R(10.0)));
+  const R ewlog2 = ewlog - R(1.3816e-07) * (pow(R(10), R(11.344) * (R(1) - (R(1) / tsot))) - R(1));
+  const R ewlog3 = ewlog2 + R(0.0081328) * (pow(R(10), R(-3.49149) * (tsot - R(1))) - R(1));
+  const R ewlog4 = ewlog3 + (log(R(1013.246)) / log(R(10.0)));
+  return pow(R(10.0), ewlog4);
+}
+
+struct MetArgs {
+  const float *swdown, *lwdown, *tair, *qair, *wind, *wind_e, *psurf, *rainf;
+  const int64_t *tix;       // device position -> caller tile
+  const double *elev, *lat; // per caller tile
+  const double *hod, *doy;  // per step of the window
+  double tair_offset, psurf_scale, rainf_scale;
+  int qair_is_rh;
+};
+
+// one window of forcing from the met variables, as get_met_data forms met%
+// (src/offline/cable_input.F90): fsd(:,1) = fsd(:,2) = SWdown/2 (:1880-1883),
+// tk = Tair + convert%Tair (:1923), pmb = PSurf * convert%PSurf (:1959) or,
+// without PSurf, from elevation and tk (:1979-1981), qv = Qair or rh_sh of
+// it (:2000-2003), ua = Wind or sqrt(Wind_N**2 + Wind_E**2) (:2094),
+// precip = Rainf * convert%Rainf (:2202-2204), fld = LWdown or Swinbank's
+// 0.0000094 * 0.0000000567 * tk**6 (:2257), coszen = sinbet(doy, lat, hod)
+// (:2677, src/science/radiation/cbl_sinbet.F90:8-28)
+template <class R>
+__global__ void __launch_bounds__(256) met_kernel(int64_t mp, MetArgs m, R *fsd1, R *fld, R *tk, R *qv, R *ua,
+                                                  R *pmb, R *precip, R *coszen) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= mp) return;
+  const int64_t t = m.tix[j], k = blockIdx.y, src = k * mp + t, o = k * mp + j;
+  const R tair = (R)m.tair[src] + (R)m.tair_offset;
+  R ps;
+  if (m.psurf) {
+    ps = (R)m.psurf[src] * (R)m.psurf_scale;
+  } else {
+    ps = R(1013.25) * pow(tair / (tair + R(0.0065) * (R)m.elev[t]), R(9.80665) / R(287.04) / R(0.0065));
+  }
+  R q = (R)m.qair[src];
+  if (m.qair_is_rh) {
+    const R es = met_svp(tair);
+    const R ws = R(0.622) * es / (ps - es);
+    q = (q / R(100.0)) * ws;
+  }
+  R u = (R)m.wind[src];
+  if (m.wind_e) {
+    const R e = (R)m.wind_e[src];
+    u = sqrt(u * u + e * e);
+  }
+  const R lw = m.lwdown ? (R)m.lwdown[src] : R(0.0000094) * R(0.0000000567) * pow(tair, R(6.0));
+  const R pi = R(3.1415927), pi180 = pi / R(180.0);
+  const R doy = (R)m.doy[k], hod = (R)m.hod[k], xslat = (R)m.lat[t];
+  const R sindec = -sin(R(23.45) * pi180) * cos(R(2.) * pi * (doy + R(10.0)) / R(365.0));
+  R z = sin(pi180 * xslat) * sindec + cos(pi180 * xslat) * sqrt(R(1.) - sindec * sindec) * cos(pi * (hod - R(12.0)) / R(12.0));
+  z = z > R(1e-8) ? z : R(1e-8);
+  fsd1[o] = R(0.5) * (R)m.swdown[src];
+  tk[o] = tair;
+  pmb[o] = ps;
+  qv[o] = q;
+  ua[o] = u;
+  precip[o] = (R)m.rainf[src] * (R)m.rainf_scale;
+  fld[o] = lw;
+  coszen[o] = z;
+}
+
+// the reader's clock (src/offline/cable_input.F90:1685-1850, local time):
+// hod = shod at the run's first step, then hod + dels/3600 per step; a day
+// rolls over at 24 h with the month and year as the reader sets them
+// (IS_LEAPYEAR, src/util/cable_common.F90:226-235)
+template <class R>
+struct MetClock {
+  R hod, doy;
+  int moy, year;
+  static bool leap(int y) { return ((y % 4 == 0 && y % 100 != 0) || y % 400 == 0); }
+  void roll() {
+    if (hod < R(0.0)) {
+      doy = doy - 1;
+      hod = hod + R(24.0);
+      static const int ml[] = {31, 60, 91, 121, 152, 182, 213, 244, 274, 305, 335};
+      static const int mn[] = {31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334};
+      const int *tab = leap(year) ? ml : mn;
+      const int d = (int)doy;
+      if (d == 0) {
+        moy = 12;
+        year = year - 1;
+        // the reader assumes the year before a leap year is not one
+        doy = leap(year + 1) ? R(365) : (leap(year) ? R(366) : R(365));
+      }
+      for (int i = 0; i < 11; i++)
+        if (d == tab[i]) moy = i + 1;
+    } else if (hod >= R(24.0)) {
+      doy = doy + 1;
+      hod = hod - R(24.0);
+      static const int ml[] = {32, 61, 92, 122, 153, 183, 214, 245, 275, 306, 336};
+      static const int mn[] = {32, 60, 91, 121, 152, 182, 213, 244, 274, 305, 335};
+      const bool lp = leap(year);
+      const int *tab = lp ? ml : mn;
+      const int d = (int)doy;
+      for (int i = 0; i < 11; i++)
+        if (d == tab[i]) moy = i + 2;
+      if (d == (lp ? 367 : 366)) {
+        year = year + 1;
+        moy = 1;
+        doy = R(1);
+      }
+    }
+  }
+};
+
+template <class R>
+void met_clock(double shod, int sdoy, int smoy, int syear, double dels, int k_first, int nsteps,
+               std::vector<double> &hod, std::vector<double> &doy) {
+  MetClock<R> c{(R)shod, (R)sdoy, smoy, syear};
+  c.roll();
+  for (int k = 0; k < k_first + nsteps; k++) {
+    if (k > 0) {
+      c.hod = c.hod + (R)dels / R(3600.0);
+      c.roll();
+    }
+    if (k >= k_first) {
+      hod.push_back((double)c.hod);
+      doy.push_back((double)c.doy);
+    }
+  }
 }
 
 // SplitMix64's increment and finaliser (Steele, Lea and Flood, "Fast
@@ -727,8 +872,110 @@ int cable_gpu_synth_forcing(cable_gpu_ctx *c, int k_first, int nsteps, uint64_t 
   HIPCHECK(hipStreamSynchronize(c->stream));
   c->nsteps_forc = nsteps;
   c->forc_k0 = k_first;
+  c->forc_doy.clear();
   c->ca = 400e-6;
   c->doy = doy;
+  return CABLE_OK;
+}
+
+int cable_gpu_set_met(cable_gpu_ctx *c, int k_first, int nsteps, double dels, const cable_gpu_met *m) {
+  if (!c || !m || k_first < 0 || nsteps <= 0 || nsteps > 65535 || !(dels > 0))
+    return fail(CABLE_E_ARG, "cable_gpu_set_met: bad arguments");
+  if (!m->swdown || !m->tair || !m->qair || !m->wind || !m->rainf || !m->latitude || (!m->psurf && !m->elevation))
+    return fail(CABLE_E_ARG, "cable_gpu_set_met: SWdown, Tair, Qair, Wind, Rainf and latitude are required, "
+                             "and elevation when PSurf is absent");
+  const size_t mp = (size_t)c->mp, n = (size_t)nsteps * mp;
+  // the clock of the window, uniform over the tiles (local time)
+  std::vector<double> hod, doy;
+  if (c->precision == CABLE_PREC_MIXED)
+    met_clock<float>(m->shod, m->sdoy, m->smoy, m->syear, dels, k_first, nsteps, hod, doy);
+  else
+    met_clock<double>(m->shod, m->sdoy, m->smoy, m->syear, dels, k_first, nsteps, hod, doy);
+  FieldMeta fm = {"forcing", 'F', 1, ROLE_FORC};
+  const size_t bytes = dev_elem(fm, c->precision) * n;
+  if (c->nsteps_forc != nsteps || c->forc.size() != 8) {
+    for (void *p : c->forc)
+      if (p) HIPCHECK(hipFree(p));
+    c->forc.assign(8, nullptr);
+    c->nsteps_forc = 0;
+    for (int i = 0; i < 8; i++) HIPCHECK(hipMalloc(&c->forc[i], bytes));
+  }
+  // inputs to the device: the met variables in the caller's tile order
+  std::vector<void *> tmp;
+  auto up = [&](const void *h, size_t nb) -> void * {
+    if (!h) return nullptr;
+    void *d = nullptr;
+    if (hipMalloc(&d, nb) != hipSuccess) return nullptr;
+    tmp.push_back(d);
+    if (hipMemcpyAsync(d, h, nb, hipMemcpyHostToDevice, c->stream) != hipSuccess) return nullptr;
+    return d;
+  };
+  std::vector<int64_t> tix(mp);
+  for (size_t j = 0; j < mp; j++) tix[j] = c->order.empty() ? (int64_t)j : c->order[j];
+  MetArgs a;
+  memset(&a, 0, sizeof a);
+  const size_t fb = sizeof(float) * n;
+  a.swdown = (const float *)up(m->swdown, fb);
+  a.lwdown = (const float *)up(m->lwdown, fb);
+  a.tair = (const float *)up(m->tair, fb);
+  a.qair = (const float *)up(m->qair, fb);
+  a.wind = (const float *)up(m->wind, fb);
+  a.wind_e = (const float *)up(m->wind_e, fb);
+  a.psurf = (const float *)up(m->psurf, fb);
+  a.rainf = (const float *)up(m->rainf, fb);
+  a.tix = (const int64_t *)up(tix.data(), sizeof(int64_t) * mp);
+  a.elev = (const double *)up(m->elevation, sizeof(double) * mp);
+  a.lat = (const double *)up(m->latitude, sizeof(double) * mp);
+  a.hod = (const double *)up(hod.data(), sizeof(double) * hod.size());
+  a.doy = (const double *)up(doy.data(), sizeof(double) * doy.size());
+  a.tair_offset = m->tair_offset;
+  a.psurf_scale = m->psurf_scale;
+  a.rainf_scale = m->rainf_scale;
+  a.qair_is_rh = m->qair_is_rh;
+  const bool ok = a.swdown && a.tair && a.qair && a.wind && a.rainf && a.tix && a.lat && a.hod && a.doy &&
+                  (!m->lwdown || a.lwdown) && (!m->wind_e || a.wind_e) && (!m->psurf || a.psurf) &&
+                  (!m->elevation || a.elev);
+  hipError_t e = hipSuccess;
+  if (ok) {
+    const dim3 grid((unsigned)((mp + 255) / 256), (unsigned)nsteps);
+    void **f = c->forc.data();
+    if (c->precision == CABLE_PREC_MIXED)
+      hipLaunchKernelGGL(met_kernel<float>, grid, dim3(256), 0, c->stream, c->mp, a, (float *)f[0], (float *)f[1],
+                         (float *)f[2], (float *)f[3], (float *)f[4], (float *)f[5], (float *)f[6], (float *)f[7]);
+    else
+      hipLaunchKernelGGL(met_kernel<double>, grid, dim3(256), 0, c->stream, c->mp, a, (double *)f[0], (double *)f[1],
+                         (double *)f[2], (double *)f[3], (double *)f[4], (double *)f[5], (double *)f[6],
+                         (double *)f[7]);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  for (void *d : tmp) hipFree(d);
+  if (!ok) return fail(CABLE_E_HIP, "cable_gpu_set_met: device buffers for the met variables");
+  if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_set_met: %s", hipGetErrorString(e));
+  const int ids[8] = {CABLE_F_met_fsd, CABLE_F_met_fld, CABLE_F_met_tk, CABLE_F_met_qv,
+                      CABLE_F_met_ua, CABLE_F_met_pmb, CABLE_F_met_precip, CABLE_F_met_coszen};
+  std::vector<void *> tab = c->dev;
+  for (int i = 0; i < 8; i++) tab[ids[i]] = c->forc[i];
+  HIPCHECK(hipMemcpyAsync(c->d_tab_run, tab.data(), sizeof(void *) * CABLE_NFIELDS, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->nsteps_forc = nsteps;
+  c->forc_k0 = k_first;
+  c->forc_doy = doy;
+  c->ca = m->fixed_co2 / 1000000.0;
+  c->doy = doy[0];
+  return CABLE_OK;
+}
+
+int cable_gpu_met_clock(int precision, double shod, int sdoy, int smoy, int syear, double dels, int k_first,
+                        int nsteps, double *hod, double *doy) {
+  if (!hod || !doy || k_first < 0 || nsteps <= 0 || !(dels > 0)) return fail(CABLE_E_ARG, "cable_gpu_met_clock: bad arguments");
+  std::vector<double> h, d;
+  if (precision == CABLE_PREC_MIXED)
+    met_clock<float>(shod, sdoy, smoy, syear, dels, k_first, nsteps, h, d);
+  else
+    met_clock<double>(shod, sdoy, smoy, syear, dels, k_first, nsteps, h, d);
+  memcpy(hod, h.data(), sizeof(double) * (size_t)nsteps);
+  memcpy(doy, d.data(), sizeof(double) * (size_t)nsteps);
   return CABLE_OK;
 }
 

@@ -35,6 +35,7 @@ EXPORTS = [
     "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
     "cable_gpu_download_forcing", "cable_gpu_surrogate_eval", "cable_gpu_consistency_check",
     "cable_gpu_consistency_values", "cable_gpu_state_bytes", "cable_gpu_save_state", "cable_gpu_load_state",
+    "cable_gpu_set_met", "cable_gpu_met_clock",
     "cable_gpu_last_error",
 ]
 
@@ -43,6 +44,16 @@ _lib = None
 
 class CableError(RuntimeError):
     pass
+
+
+class MetVars(ctypes.Structure):
+    """cable_gpu_met (include/cable_gpu.h)."""
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("swdown", "lwdown", "tair", "qair", "wind", "wind_e", "psurf", "rainf")] + [
+        ("tair_offset", ctypes.c_double), ("psurf_scale", ctypes.c_double), ("rainf_scale", ctypes.c_double),
+        ("qair_is_rh", ctypes.c_int), ("elevation", ctypes.c_void_p), ("latitude", ctypes.c_void_p),
+        ("shod", ctypes.c_double), ("sdoy", ctypes.c_int), ("smoy", ctypes.c_int), ("syear", ctypes.c_int),
+        ("fixed_co2", ctypes.c_double)]
 
 
 def lib():
@@ -89,6 +100,8 @@ def lib():
         "cable_gpu_surrogate_eval": ([vp, c_int, vp, vp], c_int),
         "cable_gpu_consistency_check": ([vp, c_int], c_int),
         "cable_gpu_state_bytes": ([vp], i64),
+        "cable_gpu_set_met": ([vp, c_int, c_int, c_double, ctypes.POINTER(MetVars)], c_int),
+        "cable_gpu_met_clock": ([c_int, c_double, c_int, c_int, c_int, c_double, c_int, c_int, vp, vp], c_int),
         "cable_gpu_save_state": ([vp, vp, i64], c_int),
         "cable_gpu_load_state": ([vp, vp, i64, c_int], c_int),
         "cable_gpu_consistency_values": ([vp, vp], c_int),
@@ -244,6 +257,38 @@ class CableGPU:
                                              float(dels), float(doy), float(hod0), _ptr(t0), _ptr(lat)))
         self.nsteps = int(nsteps)
 
+    def set_met(self, k_first: int, nsteps: int, dels: float, met: dict, latitude, elevation=None,
+                start=(0.0, 1, 1, 2000), fixed_co2=400.0):
+        """Resident forcing from met variables in the file's units
+        (cable_gpu_set_met).  met: swdown, tair, qair, wind, rainf and
+        optionally lwdown, wind_e, psurf, each (nsteps, mp), plus the
+        conversions tair_offset, psurf_scale, rainf_scale, qair_is_rh;
+        start = (shod, sdoy, smoy, syear) of the run's first step."""
+        keep = []
+
+        def arr(x, dt):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(x, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+        m = MetVars()
+        for k in ("swdown", "lwdown", "tair", "qair", "wind", "wind_e", "psurf", "rainf"):
+            v = met.get(k)
+            if v is not None and np.shape(v) != (nsteps, self.mp):
+                raise CableError(f"{k}: shape {np.shape(v)}, expected {(nsteps, self.mp)}")
+            setattr(m, k, arr(v, np.float32))
+        m.tair_offset = float(met.get("tair_offset", 0.0))
+        m.psurf_scale = float(met.get("psurf_scale", 0.01))
+        m.rainf_scale = float(met.get("rainf_scale", dels))
+        m.qair_is_rh = 1 if met.get("qair_is_rh") else 0
+        m.elevation = arr(elevation, np.float64)
+        m.latitude = arr(latitude, np.float64)
+        m.shod, m.sdoy, m.smoy, m.syear = float(start[0]), int(start[1]), int(start[2]), int(start[3])
+        m.fixed_co2 = float(fixed_co2)
+        _check(lib().cable_gpu_set_met(self.h, int(k_first), int(nsteps), float(dels), ctypes.byref(m)))
+        self.nsteps = int(nsteps)
+
     FORCING = ["met%fsd", "met%fld", "met%tk", "met%qv", "met%ua", "met%pmb", "met%precip", "met%coszen"]
 
     def download_forcing(self, k: int):
@@ -329,6 +374,15 @@ class CableGPU:
         out = np.zeros(6)
         _check(lib().cable_gpu_consistency_values(self.h, _ptr(out)))
         return dict(zip(["count_bal", "new_sumbal", "sum_fe", "sum_fpn", "ebal_tot", "wbal_tot"], out.tolist()))
+
+
+def met_clock(precision, start, dels, k_first, nsteps):
+    """(hod, doy) of steps k_first.. on the offline reader's clock, as the
+    library computes it (no device work)."""
+    hod, doy = np.empty(nsteps), np.empty(nsteps)
+    _check(lib().cable_gpu_met_clock(int(precision), float(start[0]), int(start[1]), int(start[2]),
+                                     int(start[3]), float(dels), int(k_first), int(nsteps), _ptr(hod), _ptr(doy)))
+    return hod, doy
 
 
 def class_order(iveg, isoilm=None, vlai=None):

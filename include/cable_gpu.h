@@ -165,6 +165,43 @@ int cable_gpu_set_forcing(cable_gpu_ctx *ctx, int nsteps, const void *fsd1, cons
    arrays (mp values).  cable_gpu_run then accepts steps in the window. */
 int cable_gpu_synth_forcing(cable_gpu_ctx *ctx, int k_first, int nsteps, uint64_t seed, double dels,
                             double doy, double hod0, const double *t0, const double *lat);
+/* Met forcing in the met file's own variables and units, turned into the
+   resident forcing as get_met_data forms met% (src/offline/cable_input.F90:
+   1650-2692) for the offline driver's local-time clock.  Arrays are
+   (nsteps, mp) FP32 in the caller's tile order, as a met file holds them;
+   null where the file lacks the variable.  No counterpart as a function in
+   the reference, which reads the file itself. */
+typedef struct {
+  const float *swdown;   /* W/m2; fsd(:,1) = fsd(:,2) = SWdown/2 (:1880-1883) */
+  const float *lwdown;   /* W/m2; null: Swinbank's 0.0000094*0.0000000567*tk**6 (:2257) */
+  const float *tair;     /* Tair + tair_offset = tk (:1923) */
+  const float *qair;     /* kg/kg, or relative humidity in % (qair_is_rh) through rh_sh (:2000-2003) */
+  const float *wind;     /* m/s: scalar wind, or the northward part when wind_e is given */
+  const float *wind_e;   /* m/s eastward part: ua = sqrt(wind**2 + wind_e**2) (:2094); or null */
+  const float *psurf;    /* PSurf * psurf_scale = pmb in hPa (:1959); null: from elevation and tk (:1979-1981) */
+  const float *rainf;    /* Rainf * rainf_scale = precip in mm per step (:2202-2204) */
+  double tair_offset;    /* convert%Tair: 273.16 for deg C, 0 for K (:1051-1058) */
+  double psurf_scale;    /* convert%PSurf: 0.01 for Pa, 10 for kPa, 1 for hPa (:1197-1210) */
+  double rainf_scale;    /* convert%Rainf: dels for per-second rates, dels/3600 for per-hour (:1108-1115) */
+  int qair_is_rh;        /* convert%Qair == -999 (:1075-1083) */
+  const double *elevation; /* m per tile (needed when psurf is null) */
+  const double *latitude;  /* degrees per tile, for coszen = sinbet(doy, lat, hod) (:2677) */
+  double shod;           /* local hour of day of the run's first step (ktau = kstart) */
+  int sdoy, smoy, syear; /* its day of year, month and year */
+  double fixed_co2;      /* ppmv: ca = fixedCO2/1e6 (:2295) */
+} cable_gpu_met;
+/* Steps k_first .. k_first+nsteps-1 of a run whose first step is k = 0: the
+   clock (hod, doy; :1685-1850) advances dels/3600 per step from shod and
+   rolls days, months and years over as the reader does, and the step uses
+   the window's day of year.  Snowf is not taken: precip_sn is partitioned
+   by temperature in the step (:2666-2672), the reader's rule when the file
+   has no snowfall. */
+int cable_gpu_set_met(cable_gpu_ctx *ctx, int k_first, int nsteps, double dels, const cable_gpu_met *met);
+/* The reader's clock alone, in the precision's REAL (no device work): hod
+   and doy of steps k_first .. k_first+nsteps-1. */
+int cable_gpu_met_clock(int precision, double shod, int sdoy, int smoy, int syear, double dels, int k_first,
+                        int nsteps, double *hod, double *doy);
+
 /* One resident forcing variable of step k in the caller's tile order;
    which: 0 fsd(:,1), 1 fld, 2 tk, 3 qv, 4 ua, 5 pmb, 6 precip, 7 coszen. */
 int cable_gpu_download_forcing(cable_gpu_ctx *ctx, int k, int which, void *host, int host_type);
