@@ -35,6 +35,53 @@ def test_bindings_cover_every_field():
         assert f"C_LOC({f.name})" in txt, f.name
 
 
+def _flang():
+    import shutil
+    for c in (shutil.which("amdflang"), "/opt/rocm/llvm/bin/amdflang"):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+@pytest.mark.skipif(_flang() is None, reason="no Fortran compiler")
+def test_fortran_met_type_has_the_c_layout(tmp_path):
+    """TYPE(cable_gpu_met) of the ISO_C_BINDING module and struct
+    cable_gpu_met of include/cable_gpu.h: same size and field offsets
+    (the ctypes mirror follows the C rules)."""
+    import ctypes
+    import subprocess
+    from cable_amd import gpu
+    src = tmp_path / "layout.F90"
+    src.write_text("""PROGRAM layout
+USE, INTRINSIC :: ISO_C_BINDING
+USE cable_gpu_c, ONLY: cable_gpu_met
+IMPLICIT NONE
+TYPE(cable_gpu_met), TARGET :: m
+INTEGER(C_INTPTR_T) :: b
+b = TRANSFER(C_LOC(m), b)
+WRITE(*, '(I0)') C_SIZEOF(m)
+WRITE(*, '(I0)') TRANSFER(C_LOC(m%tair_offset), b) - b
+WRITE(*, '(I0)') TRANSFER(C_LOC(m%qair_is_rh), b) - b
+WRITE(*, '(I0)') TRANSFER(C_LOC(m%latitude), b) - b
+WRITE(*, '(I0)') TRANSFER(C_LOC(m%sdoy), b) - b
+WRITE(*, '(I0)') TRANSFER(C_LOC(m%syear), b) - b
+WRITE(*, '(I0)') TRANSFER(C_LOC(m%fixed_co2), b) - b
+END PROGRAM layout
+""")
+    mod = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cable_amd", "fortran",
+                       "cable_gpu_c.F90")
+    exe = tmp_path / "layout"
+    libdir = os.path.dirname(gpu.LIBPATH)
+    gpu.lib()  # built
+    subprocess.run([_flang(), "-J", str(tmp_path), "-o", str(exe), mod, str(src), f"-L{libdir}", "-lcable_gpu",
+                    f"-Wl,-rpath,{libdir}"], check=True, cwd=tmp_path)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    M = gpu.MetVars
+    want = [ctypes.sizeof(M)] + [getattr(M, f).offset for f in
+                                 ("tair_offset", "qair_is_rh", "latitude", "sdoy", "syear", "fixed_co2")]
+    assert got == want
+
+
 def _scaled_err(got, exp, name):
     got = np.asarray(got, dtype=np.float64).reshape(-1)
     exp = np.asarray(exp, dtype=np.float64).reshape(-1)
