@@ -42,12 +42,19 @@ HEADLINE = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes",
             "ssnow%runoff", "rad%trad"]
 
 
+# compared between forms: the headline fluxes, soil temperature (K) and soil
+# moisture (m3/m3) of all six layers
+COMPARED = HEADLINE + ["ssnow%tgg", "ssnow%wb"]
+
+
 def flux_errors(got, ref):
-    """max, 99.9th and 99th percentile of |got - ref| per headline flux."""
+    """max, 99.9th and 99th percentile and RMS of |got - ref| per flux
+    (and per soil state over all layers)."""
     out = {}
-    for k in HEADLINE:
+    for k in COMPARED:
         d = np.abs(np.asarray(got[k], dtype=np.float64) - np.asarray(ref[k], dtype=np.float64)).ravel()
-        out[k] = {"max": float(d.max()), "p99.9": float(np.quantile(d, 0.999)), "p99": float(np.quantile(d, 0.99))}
+        out[k] = {"max": float(d.max()), "p99.9": float(np.quantile(d, 0.999)), "p99": float(np.quantile(d, 0.99)),
+                  "rms": float(np.sqrt(np.mean(d * d)))}
     return out
 
 
@@ -85,7 +92,7 @@ def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None, ke
         fb, ts = g.hybrid_counters()
         extra["fallback_fraction"] = fb / ts
     if keep:
-        extra["_out"] = {k: g.download(k) for k in HEADLINE}
+        extra["_out"] = {k: g.download(k) for k in COMPARED}
     g.close()
     per_launch_s = ev_ms * 1e-3 / steps
     achieved = ALGO_BYTES[precision] * n / per_launch_s / 1e9
@@ -243,7 +250,7 @@ def production(args, log):
                 f, s_ = g.hybrid_counters()
                 fb, ts = fb + f, ts + s_
         log(f"[bench] production steps {k0}..{k0 + nk - 1} done")
-    out = {k: {h: g.download(h) for h in HEADLINE} for k, g in ctx.items()}
+    out = {k: {h: g.download(h) for h in COMPARED} for k, g in ctx.items()}
     for g in ctx.values():
         g.close()
     forms = {}
