@@ -58,6 +58,24 @@ def flux_errors(got, ref):
     return out
 
 
+_COSTS = {}
+
+
+def class_costs(wl, prec, log):
+    """Per-vegetation-type step cost for the longest-first tile order,
+    measured once per precision (outside every timed region)."""
+    from cable_amd import gpu
+    if prec not in _COSTS:
+        t = time.perf_counter()
+        _COSTS[prec] = gpu.class_costs(wl, prec)
+        log(f"[bench] class costs ({'fp64' if prec == gpu.PREC_FP64 else 'mixed'}) measured in "
+            f"{time.perf_counter() - t:.1f}s")
+    return _COSTS[prec]
+
+
+ORDER_LABEL = "grouped by vegetation type (measured most expensive first), then soil type and leaf area"
+
+
 def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None, keep=False):
     from cable_amd import gpu
     from cable_amd.workload import Workload
@@ -66,7 +84,7 @@ def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None, ke
     wl = Workload(n, nsteps)
     log(f"[bench] workload {n} tiles x {nsteps} steps generated in {time.perf_counter() - t:.1f}s")
     prec = gpu.PREC_FP64 if precision == "fp64" else gpu.PREC_MIXED
-    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai) if class_order else None
+    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(wl, prec, log)) if class_order else None
 
     def go(g, k):
         if hybrid is None:
@@ -220,10 +238,12 @@ def production(args, log):
     wl = Workload(n, 1, forcing=False)
     log(f"[bench] production workload {n} tiles generated in {time.perf_counter() - t:.1f}s")
     params, norm, info = surrogate.train_on_gpu(log=log, **TRAIN)
-    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai)
-    ctx = {"fp64": gpu.from_workload(wl, precision=gpu.PREC_FP64, full=False, order=order),
-           "mixed": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order),
-           "hybrid": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order)}
+    cal = Workload(65536, 8)        # calibration of the longest-first order
+    o64 = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(cal, gpu.PREC_FP64, log))
+    o32 = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(cal, gpu.PREC_MIXED, log))
+    ctx = {"fp64": gpu.from_workload(wl, precision=gpu.PREC_FP64, full=False, order=o64),
+           "mixed": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=o32),
+           "hybrid": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=o32)}
     ctx["hybrid"].set_surrogate(params, norm)
     wall = {k: 0.0 for k in ctx}
     dev_ms = {k: 0.0 for k in ctx}
@@ -275,7 +295,7 @@ def production(args, log):
         "data": "synthetic (forcing generated on the device, parameters from the reference's PFT and soil tables)",
         "config": {"workload": f"hybrid production run, {n} tiles x {total} half-hourly steps",
                    "tiles": n, "precision": "mixed + BF16 surrogate warm start", "path": "device-resident run",
-                   "tile_order": "grouped by vegetation and soil type, then leaf area", "forcing_window_steps": win},
+                   "tile_order": ORDER_LABEL, "forcing_window_steps": win},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,
@@ -373,7 +393,7 @@ def main():
         "data": "synthetic (seeded forcing, parameters from the reference's PFT and soil tables)",
         "config": {"workload": f"{what}, {args.tiles} tiles x {args.steps} half-hourly steps",
                    "tiles": args.tiles, "precision": prec, "path": "device-resident run",
-                   "tile_order": "caller" if args.caller_order else "grouped by vegetation and soil type, then leaf area"},
+                   "tile_order": "caller" if args.caller_order else ORDER_LABEL},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src},

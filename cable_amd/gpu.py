@@ -385,16 +385,39 @@ def met_clock(precision, start, dels, k_first, nsteps):
     return hod, doy
 
 
-def class_order(iveg, isoilm=None, vlai=None):
+def class_order(iveg, isoilm=None, vlai=None, cost=None):
     """A tile order that groups tiles by vegetation type (then soil type, then
     leaf area): the wavefronts then run one type's canopy iteration counts and
-    branches, and tiles of similar leaf area side by side."""
+    branches, and tiles of similar leaf area side by side.  cost: optional
+    {vegetation type: time per tile-step} (class_costs); the types then come
+    most expensive first, so that the last workgroups the dispatcher starts
+    are the short ones (longest-first scheduling)."""
     iveg = np.asarray(iveg)
     if isoilm is None:
         return np.argsort(iveg, kind="stable")
     if vlai is None:
         return np.lexsort((np.asarray(isoilm), iveg))
-    return np.lexsort((np.asarray(vlai), np.asarray(isoilm), iveg))
+    if cost is None:
+        return np.lexsort((np.asarray(vlai), np.asarray(isoilm), iveg))
+    rank = {c: r for r, c in enumerate(sorted(cost, key=lambda c: (-cost[c], c)))}
+    key = np.array([rank.get(int(c), len(rank)) for c in iveg])
+    return np.lexsort((np.asarray(vlai), np.asarray(isoilm), iveg, key))
+
+
+def class_costs(wl, precision=PREC_FP64, sample=4096, steps=8):
+    """Measured device time per tile-step of each vegetation type of workload
+    wl: the resident run on up to `sample` tiles of the type for `steps`
+    steps (a calibration for class_order)."""
+    out = {}
+    for c in np.unique(wl.iveg):
+        idx = np.nonzero(wl.iveg == c)[0][:sample]
+        sub = wl.subset(idx)
+        nst = min(steps, sub.nsteps)
+        g = from_workload(sub, precision=precision, full=False)
+        g.run(0, nst, sub.dels)
+        out[int(c)] = g.last_run_ms() / (nst * sub.n)
+        g.close()
+    return out
 
 
 def from_workload(wl, precision=PREC_FP64, full=True, order=None):

@@ -102,3 +102,19 @@ def test_new_entry_points_reject_bad_arguments(L):
     assert b"bad arguments" in L.cable_gpu_last_error()
     assert L.cable_gpu_create(ctypes.byref(ctypes.c_void_p()), 16, 7) == 1
     assert b"precision" in L.cable_gpu_last_error()
+
+
+def test_class_order_is_a_permutation_grouped_by_type():
+    from cable_amd.workload import Workload
+    wl = Workload(3000, 1, forcing=False)
+    cost = {c: float(c % 4) for c in range(1, 18)}
+    for o in (gpu.class_order(wl.iveg, wl.isoilm, wl.vlai), gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, cost)):
+        assert sorted(o.tolist()) == list(range(wl.n))
+        v = wl.iveg[o]
+        # each vegetation type is one contiguous run
+        runs = v[np.r_[True, v[1:] != v[:-1]]]
+        assert len(runs) == len(set(runs.tolist()))
+    # with costs, the types come most expensive first (ties by type)
+    v = wl.iveg[gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, cost)]
+    runs = v[np.r_[True, v[1:] != v[:-1]]].tolist()
+    assert runs == sorted(runs, key=lambda c: (-cost[c], c))
