@@ -334,8 +334,8 @@ def main():
                           "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        sys.stderr.write("bench.py: this build measures one GPU only (tiles are independent; "
-                         "sharding across GPUs is not part of it). Run with --gpus 1.\n")
+        sys.stderr.write("bench.py: the library steps the tiles of one GPU; sharding across GPUs is not "
+                         "implemented (tiles are independent). Run with --gpus 1.\n")
         sys.exit(2)
 
     def log(msg):
