@@ -1,5 +1,6 @@
-// cbm_kernels_mixed.hip -- kernels of the mixed path: default-REAL quantities in FP32, REAL(r_2) and the
-// solves in FP64.  Compiled with FMA contraction inside expressions and the
+// cbm_kernels_mixed.hip -- kernels of the mixed path: radiation, roughness and
+// the canopy iteration in FP32; the soil water and heat state, the two
+// tridiagonal solves and the balance accumulators in FP64.  Compiled with FMA contraction inside expressions and the
 // hardware approximations of the elementary functions: the path is held to
 // the flux tolerance of its contract, not to the reference's rounding.
 #include "cbm_kernel.h"

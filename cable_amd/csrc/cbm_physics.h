@@ -6,10 +6,12 @@
 // state, parameters and forcing and to write its state and outputs.
 //
 // The code is a template on
-//   R     the precision of the reference's default-REAL quantities: double for
-//         the plain FP64 path, float for the mixed path.  REAL(r_2) state (soil
-//         moisture, ice, heat capacity), the two tridiagonal (Thomas) solves and
-//         the balance accumulators are double in both paths.
+//   R     the working precision: double for the plain FP64 path, float for
+//         the mixed path.  Default-REAL quantities, the whole canopy iteration
+//         (its REAL(r_2) quantities included) and the soil's elementary
+//         functions are computed in R.  REAL(r_2) state (soil moisture, ice,
+//         heat capacity) and its updates, the two tridiagonal (Thomas) systems
+//         and the balance accumulators are double in both paths.
 //   FULL  write every diagnostic of the reference's derived types (the drop-in
 //         mode used by the Fortran cbm() replacement and the parity tests), or
 //         only the prognostic state and the eight headline fluxes.

@@ -483,21 +483,25 @@ struct Stepper {
     return MIN(MIN(anrubiscoz, anrubpz), ansinkz);
   }
 
-  // src/science/soilsnow/cbl_remove_trans.F90:43-93
-  CBM_HD void transp_soil_water(R fevc, double *evapfbl) {
-    double dels = a.dels;
-    double diff_prev = 0.0;
+  // src/science/soilsnow/cbl_remove_trans.F90:43-93.  The canopy iteration's
+  // REAL(r_2) quantities (transpiration per layer, fevc, fess, dgdtg) are
+  // computed in the step's working precision R: FP64 in the plain path, FP32
+  // in the mixed path, whose FP64 is kept to the tridiagonal solves and the
+  // balance accumulators.  avail[k], the water layer k can give
+  // (MAX(0, wbliq - 1.1 swilt) zse 1000), does not change during the canopy
+  // iteration and is computed once per step (define_canopy).
+  CBM_HD void transp_soil_water(R fevc, const R *avail, R *evapfbl) {
+    R diff_prev = K(0.0);
 #pragma unroll
     for (int k = 0; k < MS; k++) {
-      double xx = (double)fevc * dels / (double)HL * (double)PV(veg_froot, k) + diff_prev;
-      double dk = MAX(0.0, t.ssnow_wbliq[k] - 1.1 * PV(soil_swilt_vec, k)) * PV(soil_zse_vec, k) * 1000.0;
-      double xxd = xx - dk;
-      if (xxd > 0.0) {
-        evapfbl[k] = dk;
+      R xx = fevc * (R)a.dels / HL * PV(veg_froot, k) + diff_prev;
+      R xxd = xx - avail[k];
+      if (xxd > K(0.0)) {
+        evapfbl[k] = avail[k];
         diff_prev = xxd;
       } else {
         evapfbl[k] = xx;
-        diff_prev = 0.0;
+        diff_prev = K(0.0);
       }
     }
   }
@@ -507,6 +511,7 @@ struct Stepper {
     R cansat, dsx, fwsoil, tlfx, tlfy, ecy, hcy, rny;
     R gbhu[2], gbhf[2], csx[2], ghwet;
     R sum_rad_rniso, sum_rad_gradis;
+    R avail[MS];   // water each soil layer can give to transpiration, mm
   };
 
   // src/science/canopy/cbl_dryLeaf.F90:10-666
@@ -521,8 +526,9 @@ struct Stepper {
       R s = K(0.0);
 #pragma unroll
       for (int k = 0; k < MS; k++) {
-        double r = (t.ssnow_wbliq[k] - PV(soil_swilt_vec, k)) / (PV(soil_sfc_vec, k) - PV(soil_swilt_vec, k));
-        R term = PV(veg_froot, k) * MAX(K(1.0e-9), MIN(K(1.0), (R)r));
+        const R swilt = (R)PV(soil_swilt_vec, k);
+        R r = ((R)t.ssnow_wbliq[k] - swilt) / ((R)PV(soil_sfc_vec, k) - swilt);
+        R term = PV(veg_froot, k) * MAX(K(1.0e-9), MIN(K(1.0), r));
         s = (k == 0) ? term : s + term;
       }
       R rwater = MAX(K(1.0e-9), s);
@@ -544,10 +550,10 @@ struct Stepper {
     R psycst[2] = {t.air_psyc[0], t.air_psyc[0]};
     R vcmxt3[2] = {K(0), K(0)}, vcmxt4[2] = {K(0), K(0)}, vx3[2] = {K(0), K(0)}, vx4[2] = {K(0), K(0)};
     R cx1 = K(0.0), cx2 = K(0.0), deltlf = K(0.0);
-    double oldevapfbl[MS];
-    t.canopy_fevc[0] = 0.0;
+    // canopy%fevc and ssnow%evapfbl while the passes run; stored at the end
+    R fevc = K(0.0), evapfbl[MS], oldevapfbl[MS];
 #pragma unroll
-    for (int k = 0; k < MS; k++) t.ssnow_evapfbl[k] = 0.0;
+    for (int k = 0; k < MS; k++) evapfbl[k] = K(0.0);
     c.ghwet = K(1.0e-3);
     t.canopy_fevw[0] = K(0.0);
     t.canopy_fhvw[0] = K(0.0);
@@ -645,12 +651,12 @@ struct Stepper {
         R fwet = t.canopy_fwet[0];
         R local_fevc = (K(1.0) - fwet) * ecx;
         if (local_fevc > K(0.0)) {
-          transp_soil_water(local_fevc, t.ssnow_evapfbl);
-          double s = t.ssnow_evapfbl[0];
+          transp_soil_water(local_fevc, c.avail, evapfbl);
+          R s = evapfbl[0];
 #pragma unroll
-          for (int q = 1; q < MS; q++) s = s + t.ssnow_evapfbl[q];
-          t.canopy_fevc[0] = s * (double)rlam / (double)a.dels;
-          ecx = (R)(t.canopy_fevc[0] / (double)(K(1.0) - fwet));
+          for (int q = 1; q < MS; q++) s = s + evapfbl[q];
+          fevc = s * rlam / (R)a.dels;
+          ecx = fevc / (K(1.0) - fwet);
         }
         hcx = (c.sum_rad_rniso - ecx - cr * dtv * c.sum_rad_gradis) * (gh[0] + gh[1]) / (ghr[0] + ghr[1]);
         c.tlfx = tvair + hcx / (cr * (gh[0] + gh[1]));
@@ -671,7 +677,7 @@ struct Stepper {
         an_y[0] = anx[0];
         an_y[1] = anx[1];
 #pragma unroll
-        for (int q = 0; q < MS; q++) oldevapfbl[q] = t.ssnow_evapfbl[q];
+        for (int q = 0; q < MS; q++) oldevapfbl[q] = evapfbl[q];
       }
       if (abs_deltlf > K(0.1)) {
         R w = K(0.5) * ((R)(k - 5 > 0 ? k - 5 : 0) / ((R)k - K(4.9999)));
@@ -687,7 +693,7 @@ struct Stepper {
         an_y[0] = anx[0];
         an_y[1] = anx[1];
 #pragma unroll
-        for (int q = 0; q < MS; q++) oldevapfbl[q] = t.ssnow_evapfbl[q];
+        for (int q = 0; q < MS; q++) oldevapfbl[q] = evapfbl[q];
       }
     }
     if constexpr (MODE == MODE_HYBRID) {
@@ -701,18 +707,21 @@ struct Stepper {
       }
     }
     R fwet = t.canopy_fwet[0];
-    t.canopy_fevc[0] = (K(1.0) - fwet) * c.ecy;
+    fevc = (K(1.0) - fwet) * c.ecy;
     if (c.ecy > K(0.0) && fwet < K(1.0)) {
       if (fabs(c.ecy - ecx) > K(1.0e-6)) {
-        double s = oldevapfbl[0];
+        R s = oldevapfbl[0];
 #pragma unroll
         for (int q = 1; q < MS; q++) s = s + oldevapfbl[q];
-        if (!(fabs(t.canopy_fevc[0] - (s * (double)rlam / (double)a.dels)) > 1.0e-4)) {
+        if (!(fabs(fevc - s * rlam / (R)a.dels) > K(1.0e-4))) {
 #pragma unroll
-          for (int q = 0; q < MS; q++) t.ssnow_evapfbl[q] = oldevapfbl[q];
+          for (int q = 0; q < MS; q++) evapfbl[q] = oldevapfbl[q];
         }
       }
     }
+    t.canopy_fevc[0] = fevc;
+#pragma unroll
+    for (int q = 0; q < MS; q++) t.ssnow_evapfbl[q] = evapfbl[q];
     t.canopy_frday[0] = K(12.0) * (rdy[0] + rdy[1]);
     t.canopy_fpn[0] = MIN(-K(12.0) * (an_y[0] + an_y[1]), t.canopy_frday[0]);
   }
@@ -758,37 +767,37 @@ struct Stepper {
     R potev = t.ssnow_potev[0], rlam = t.air_rlam[0], snowd = t.ssnow_snowd[0];
     if (potev < K(0.)) t.ssnow_wetfac[0] = K(1.0);
     R wetfac = t.ssnow_wetfac[0];
-    double fess = (double)(wetfac * potev);
+    R fess = wetfac * potev;
     R pudsto = t.ssnow_pudsto[0];
     R pwet = MAX(K(0.), MIN(K(0.2), pudsto / MAX(K(1.), (R)PV(ssnow_pudsmx, 0))));
-    fess = fess * (double)(K(1.) - pwet);
+    fess = fess * (K(1.) - pwet);
     R frescale = (R)a.zse[0] * DENSITY_LIQ * rlam / dels;
-    if (snowd < K(0.1) && fess > 0.) {
+    if (snowd < K(0.1) && fess > K(0.)) {
       R flower = (R)t.ssnow_wb[0] - PV(soil_swilt, 0) / K(2.0);
       R fupper = MAX(K(0.), flower * frescale - (R)t.ssnow_evapfbl[0] * rlam / dels);
-      fess = MIN(fess, (double)fupper);
+      fess = MIN(fess, fupper);
       fupper = (R)(t.ssnow_wb[0] - t.ssnow_wbice[0] / (double)FROZEN_LIMIT) * frescale;
       fupper = MAX(fupper, K(0.));
-      fess = MIN(fess, (double)fupper);
+      fess = MIN(fess, fupper);
     }
     R cls = K(1.);
     if (snowd >= K(0.1)) {
       cls = K(1.1335);
-      fess = (double)(cls * potev);
+      fess = cls * potev;
     }
     if (snowd < K(0.1) && potev < K(0.) && t.ssnow_tss[0] < TFRZ) {
       cls = K(1.1335);
-      fess = (double)(cls * potev);
+      fess = cls * potev;
     }
     if (snowd >= K(0.1) && potev > K(0.)) {
       cls = K(1.1335);
-      fess = (double)MIN((wetfac * potev) * cls, snowd / dels * rlam * cls);
+      fess = MIN((wetfac * potev) * cls, snowd / dels * rlam * cls);
     }
     t.ssnow_cls[0] = cls;
     R fesp = MIN(pudsto / dels * rlam, MAX(pwet * potev, K(0.)));
     t.canopy_fess[0] = fess;
     t.canopy_fesp[0] = fesp;
-    t.canopy_fes[0] = fess + (double)fesp;
+    t.canopy_fes[0] = fess + fesp;
   }
 
   // src/science/canopy/cbl_within_canopy.F90:10-159
@@ -808,7 +817,7 @@ struct Stepper {
       R dmch = e1 * rt0 * rt1 * (t.canopy_fhv[0] + t.canopy_fhs[0]) / (rho * CAPP);
       R dmae = (-epsi * CAPP / rlam) * (rt0 * rt1) * (rrbw * rrsw);
       R dmbe = (rt0 + fix_eqn * rt1) * e1 + (rt0 * rt1) * (rrbw * rrsw);
-      R dmce = e1 * rt0 * rt1 * ((R)t.canopy_fev[0] + (R)(t.canopy_fes[0] / (double)t.ssnow_cls[0])) / (rho * rlam);
+      R dmce = e1 * rt0 * rt1 * ((R)t.canopy_fev[0] + (R)t.canopy_fes[0] / t.ssnow_cls[0]) / (rho * rlam);
       R tk = t.met_tk[0], tss = t.ssnow_tss[0];
       R det = dmah * dmbe - dmae * dmbh + K(1.0e-12);
       R tvair = tk + (dmbe * dmch - dmbh * dmce) / det;
@@ -860,10 +869,10 @@ struct Stepper {
       wf = MIN(K(1.0), wf);
       wf = MAX(K(0.0), wf);
       if (t.ssnow_wbice[0] > 0.0) {
-        double ice_ratio = p2(t.ssnow_wbice[0] / t.ssnow_wb[0]);
-        double ice_factor = 1.0 - MIN(0.2, ice_ratio);
-        ice_factor = MAX(0.5, ice_factor);
-        wf = wf * (R)ice_factor;
+        R ice_ratio = p2((R)(t.ssnow_wbice[0] / t.ssnow_wb[0]));
+        R ice_factor = K(1.0) - MIN(K(0.2), ice_ratio);
+        ice_factor = MAX(K(0.5), ice_factor);
+        wf = wf * ice_factor;
       }
       if (t.ssnow_snowd[0] > K(0.1)) wf = K(0.9);
       if (t.veg_iveg[0] == LAKES) {
@@ -878,6 +887,10 @@ struct Stepper {
     c.gbhu[0] = c.gbhu[1] = K(1e-3);
 #pragma unroll
     for (int k = 0; k < MS; k++) t.ssnow_evapfbl[k] = 0.0, t.ssnow_rex[k] = 0.0;
+    // the water each soil layer can give to transpiration this step
+#pragma unroll
+    for (int k = 0; k < MS; k++)
+      c.avail[k] = (R)(MAX(0.0, t.ssnow_wbliq[k] - 1.1 * PV(soil_swilt_vec, k)) * PV(soil_zse_vec, k) * 1000.0);
     c.csx[0] = c.csx[1] = t.met_ca[0];
     t.met_tvair[0] = t.met_tk[0];
     t.met_qvair[0] = t.met_qv[0];
@@ -977,7 +990,7 @@ struct Stepper {
         }
       }
       wetLeaf(c);
-      t.canopy_fev[0] = (R)(t.canopy_fevc[0] + (double)t.canopy_fevw[0]);
+      t.canopy_fev[0] = (R)t.canopy_fevc[0] + t.canopy_fevw[0];
       R fwet = t.canopy_fwet[0];
       t.canopy_fhv[0] = (K(1.0) - fwet) * c.hcy + t.canopy_fhvw[0];
       t.canopy_fnv[0] = (K(1.0) - fwet) * c.rny + t.canopy_fevw[0] + t.canopy_fhvw[0];
@@ -1140,7 +1153,7 @@ struct Stepper {
     t.ssnow_ddq_dtg[0] = (RMH2O / RMAIR) / t.met_pmb[0] * TETENA * TETENB * TETENC / (p2(TETENC + tssn - TFRZ)) *
                          exp(TETENB * (tssn - TFRZ) / (TETENC + tssn - TFRZ));
     t.ssnow_dfe_dtg[0] = t.ssnow_dfe_ddq[0] * t.ssnow_ddq_dtg[0];
-    t.canopy_dgdtg[0] = (double)t.ssnow_dfn_dtg[0] - (double)t.ssnow_dfh_dtg[0] - (double)t.ssnow_dfe_dtg[0];
+    t.canopy_dgdtg[0] = t.ssnow_dfn_dtg[0] - t.ssnow_dfh_dtg[0] - t.ssnow_dfe_dtg[0];
     if constexpr (FULL) {
       R dtl = c.tlfy - t.met_tk[0];
       t.bal_drybal[0] = (c.ecy + c.hcy) - c.sum_rad_rniso + CAPP * RMAIR * dtl * c.sum_rad_gradis;
@@ -1464,11 +1477,12 @@ struct Stepper {
       } else {
         double ew = (double)(R)(t.ssnow_wblf[k] * ssat);
         double exp_arg = (ew * l60) + (t.ssnow_wbfice[k] * ssat * l250);
-        double f = MAX(1.0, sqrt(MIN(2.0, 0.5 * ssat / MIN(ew, 0.5 * ssat))));
+        // the conductivity's elementary functions in the working precision
+        double f = MAX(1.0, (double)sqrt(MIN(K(2.0), (R)(0.5 * ssat) / (R)MIN(ew, 0.5 * ssat))));
         if (exp_arg > 30)
           ccnsw[k] = 1.5 * f;
         else
-          ccnsw[k] = MIN(cnsd * exp(exp_arg), 1.5) * f;
+          ccnsw[k] = MIN(cnsd * (double)exp((R)exp_arg), 1.5) * f;
       }
     }
     int isflag = t.ssnow_isflag[0];
@@ -1609,6 +1623,9 @@ struct Stepper {
     double at[MS], bt[MS], ct[MS], fluxh[MS + 1], delt[MS + 1], dtt[MS + 1];
     double *wb = t.ssnow_wb, *wbice = t.ssnow_wbice, *wblf = t.ssnow_wblf;
     double ssat = PV(soil_ssat, 0), hyds = PV(soil_hyds, 0);
+    // the power laws of the hydraulic conductivity and diffusivity are
+    // evaluated in the working precision; the water state, the fluxes and the
+    // tridiagonal system are FP64
     double e1 = (double)PV(soil_i2bp3, 0) - 1;
 #pragma unroll
     for (int k = 0; k < MS; k++) at[k] = 0.0, bt[k] = 1.0, ct[k] = 0.0;
@@ -1622,7 +1639,7 @@ struct Stepper {
       delt[k] = wbl_kp - wbl_k;
       double wh = MIN(wbl_k, wbl_kp);
       if (wbice[k - 1] > 0.05 || wbice[k] > 0.01) wh = 0.9 * wbl_k + 0.1 * wbl_kp;
-      double speed_k = hyds * pow(wh / ssat, e1);
+      double speed_k = hyds * (double)pow((R)(wh / ssat), (R)e1);
       double rat = delt[k - 1] / (delt[k] + copysign(1.0e-20, delt[k]));
       double phi = MAX(MAX(0.0, MIN(1.0, 2.0 * rat)), MIN(2.0, rat));
       double fluxhi = wh, fluxlo = wbl_k;
@@ -1634,7 +1651,7 @@ struct Stepper {
       double wbl_kp = MAX(0.001, ssat - wbice[MS - 1]);
       double wh = MIN(wbl_k, wbl_kp);
       if (wbice[MS - 1] > 0.05) wh = 0.9 * wbl_k + 0.1 * wbl_kp;
-      double speed_k = hyds * pow(wh / ssat, e1);
+      double speed_k = hyds * (double)pow((R)(wh / ssat), (R)e1);
       speed_k = 0.5 * speed_k / (1. - MIN(0.5, 10. * wbice[MS - 1]));
       double fluxlo = wbl_k;
       speed_k = MIN(0.5 * speed_k, 0.5 * zse[MS - 1] / dels);
@@ -1654,7 +1671,7 @@ struct Stepper {
 #pragma unroll
     for (int k = 2; k <= MS; k++) {
       double wbh_k = (zse[k - 1] * wblf[k - 2] + zse[k - 2] * wblf[k - 1]) / (zse[k - 1] + zse[k - 2]);
-      double fact = pow(wbh_k, e2);
+      double fact = (double)pow((R)wbh_k, (R)e2);
       double pwb_wbh =
           (hsbh * (1. - MIN(2. * MIN(0.1, MAX(wbice[k - 2] / MAX(0.01, wb[k - 2]), wbice[k - 1] / MAX(0.01, wb[k - 1]))),
                             0.1))) *
@@ -1991,7 +2008,7 @@ struct Stepper {
     soil_snow();
     snow_aging();
     t.ssnow_deltss[0] = t.ssnow_tss[0] - t.ssnow_otss[0];
-    t.canopy_fev[0] = (R)(t.canopy_fevc[0] + (double)t.canopy_fevw[0]);
+    t.canopy_fev[0] = (R)t.canopy_fevc[0] + t.canopy_fevw[0];
     t.canopy_fe[0] = (R)((double)t.canopy_fev[0] + t.canopy_fes[0]);
     t.canopy_rnet[0] = t.canopy_fns[0] + t.canopy_fnv[0];
     R transd = t.rad_transd[0];
