@@ -277,7 +277,24 @@ cbm_kernel(StepArgs a) {
         for (int c = 0; c < mlp::NOUT; c++) S.pred[c] = a.pred[t * mlp::NOUT + c];
       }
     }
-    S.run(late);
+    // the offline driver's running carbon sums (driver mode): the resident
+    // run reads the previous step's sums only after soil and snow
+    auto tail = [&]() {
+      if constexpr (!FULL) {
+        if (a.driver) {
+          void *const *F = a.F;
+          using DT = typename DevT<R, 'F'>::type;
+          __asm__ __volatile__("" ::: "memory");
+#define LDF_(id) T.id[0] = ld<DT>(F[CABLE_F_##id], t);
+          LDF_(sum_flux_sumpn) LDF_(sum_flux_sumrp) LDF_(sum_flux_sumrpw) LDF_(sum_flux_sumrpr)
+          LDF_(sum_flux_sumrs) LDF_(sum_flux_sumrd) LDF_(sum_flux_dsumpn) LDF_(sum_flux_dsumrp)
+          LDF_(sum_flux_dsumrd)
+#undef LDF_
+        }
+      }
+    };
+    S.run(late, tail);
+    if (a.driver) S.sumcflux();
     passes = S.npass;
     if constexpr (CONS) {
       cons[0] = (double)T.bal_wbal[0];
@@ -336,6 +353,11 @@ cbm_kernel(StepArgs a) {
       STD_(bal_ebal_tot, 1) STD_(bal_wbal_tot, 1) STD_(bal_precip_tot, 1) STD_(bal_rnoff_tot, 1)
       STD_(bal_evap_tot, 1) STD_(bal_radbalsum, 1)
       st<double>(F[CABLE_F_ssnow_gammzz], t, T.ssnow_gammzz[0]);
+      if (a.driver) {
+        STS_(sum_flux_sumpn, 1) STS_(sum_flux_sumrp, 1) STS_(sum_flux_sumrpw, 1) STS_(sum_flux_sumrpr, 1)
+        STS_(sum_flux_sumrs, 1) STS_(sum_flux_sumrd, 1) STS_(sum_flux_dsumpn, 1) STS_(sum_flux_dsumrp, 1)
+        STS_(sum_flux_dsumrd, 1)
+      }
       // the eight headline fluxes
       STS_(canopy_rnet, 1) STS_(canopy_fe, 1) STS_(canopy_fh, 1) STS_(canopy_fev, 1)
       STD_(canopy_fes, 1) STS_(canopy_fgpp, 1) STS_(ssnow_runoff, 1) STS_(rad_trad, 1)

@@ -1972,14 +1972,16 @@ struct Stepper {
   // 725-729) and the balance accumulators (src/offline/cable_checks.F90
   // :472-618) in FP64
   CBM_HD void run() {
-    run([] {});
+    run([] {}, [] {});
   }
   // `late` runs after the canopy and before soil and snow: the resident
   // kernel loads there the state that only soil, snow, carbon and the
   // balances read, so it is not held in registers through the canopy
-  // iteration
-  template <class Late>
-  CBM_HD void run(Late &&late) {
+  // iteration.  `tail` runs after soil and snow, before the flux totals and
+  // carbon: the resident kernel issues there the loads of the driver's
+  // running sums, which sumcflux reads after the step.
+  template <class Late, class Tail>
+  CBM_HD void run(Late &&late, Tail &&tail) {
     R dels = (R)a.dels;
     if (a.driver) {
       // the offline driver before cbm, src/offline/cable_serial.F90:572-575
@@ -2007,6 +2009,7 @@ struct Stepper {
     t.ssnow_owetfac[0] = t.ssnow_wetfac[0];
     soil_snow();
     snow_aging();
+    tail();
     t.ssnow_deltss[0] = t.ssnow_tss[0] - t.ssnow_otss[0];
     t.canopy_fev[0] = (R)t.canopy_fevc[0] + t.canopy_fevw[0];
     t.canopy_fe[0] = (R)((double)t.canopy_fev[0] + t.canopy_fes[0]);
@@ -2071,6 +2074,39 @@ struct Stepper {
       }
       t.bal_ebal_tot[0] = t.bal_ebal_tot[0] + (double)ebal;
       t.bal_radbalsum[0] = t.bal_radbalsum[0] + (double)radbal;
+    }
+  }
+
+  // the offline driver's running carbon sums after the step (driver mode):
+  // sumcflux, src/offline/casa_cable.F90:482-560 with icycle = 0 and
+  // kstart = 1, called at src/offline/cable_serial.F90:713-715.  The fnee it
+  // sets is cbm's own (run above).  The caller has the previous step's sums
+  // in t (the resident run loads them in run's `tail`).
+  CBM_HD void sumcflux() {
+    const R dels = (R)a.dels;
+    const R pn = t.canopy_fpn[0] * dels, rd = t.canopy_frday[0] * dels;
+    const R rpw = t.canopy_frpw[0] * dels, rpr = t.canopy_frpr[0] * dels;
+    const R rp = t.canopy_frp[0] * dels, rs = t.canopy_frs[0] * dels;
+    if (a.ktau == 1) {
+      t.sum_flux_sumpn[0] = pn;
+      t.sum_flux_sumrd[0] = rd;
+      t.sum_flux_dsumpn[0] = pn;
+      t.sum_flux_dsumrd[0] = rd;
+      t.sum_flux_sumrpw[0] = rpw;
+      t.sum_flux_sumrpr[0] = rpr;
+      t.sum_flux_sumrp[0] = rp;
+      t.sum_flux_dsumrp[0] = rp;
+      t.sum_flux_sumrs[0] = rs;
+    } else {
+      t.sum_flux_sumpn[0] = t.sum_flux_sumpn[0] + pn;
+      t.sum_flux_sumrd[0] = t.sum_flux_sumrd[0] + rd;
+      t.sum_flux_dsumpn[0] = t.sum_flux_dsumpn[0] + pn;
+      t.sum_flux_dsumrd[0] = t.sum_flux_dsumrd[0] + rd;
+      t.sum_flux_sumrpw[0] = t.sum_flux_sumrpw[0] + rpw;
+      t.sum_flux_sumrpr[0] = t.sum_flux_sumrpr[0] + rpr;
+      t.sum_flux_sumrp[0] = t.sum_flux_sumrp[0] + rp;
+      t.sum_flux_dsumrp[0] = t.sum_flux_dsumrp[0] + rp;
+      t.sum_flux_sumrs[0] = t.sum_flux_sumrs[0] + rs;
     }
   }
 };

@@ -123,6 +123,19 @@
   /* total soil water at the previous balance check (the SAVEd owb of     \
      src/offline/cable_checks.F90:496-513), one value per tile */          \
   X(drv_owb, "drv%owb", D, 1, STATE)                                       \
+  /* the offline driver's running carbon sums, each flux times the step   \
+     length (sumcflux, src/offline/casa_cable.F90:482-560, called at      \
+     src/offline/cable_serial.F90:713-715); dsumrs, sumxrp and sumxrs are \
+     not updated there */                                                  \
+  X(sum_flux_sumpn, "sum_flux%sumpn", F, 1, STATE)                         \
+  X(sum_flux_sumrp, "sum_flux%sumrp", F, 1, STATE)                         \
+  X(sum_flux_sumrpw, "sum_flux%sumrpw", F, 1, STATE)                       \
+  X(sum_flux_sumrpr, "sum_flux%sumrpr", F, 1, STATE)                       \
+  X(sum_flux_sumrs, "sum_flux%sumrs", F, 1, STATE)                         \
+  X(sum_flux_sumrd, "sum_flux%sumrd", F, 1, STATE)                         \
+  X(sum_flux_dsumpn, "sum_flux%dsumpn", F, 1, STATE)                       \
+  X(sum_flux_dsumrp, "sum_flux%dsumrp", F, 1, STATE)                       \
+  X(sum_flux_dsumrd, "sum_flux%dsumrd", F, 1, STATE)                       \
   /* ---- diagnostics ---- */                                             \
   X(met_qvair, "met%qvair", F, 1, DIAG)                                    \
   X(met_dva, "met%dva", F, 1, DIAG)                                        \

@@ -1984,6 +1984,35 @@ static void cbm_tile(Tile *t, const Globals *g, int ktau, double dels, int first
   t->ssnow_rnof1[0] = t->ssnow_rnof1[0] * dels;
   t->ssnow_rnof2[0] = t->ssnow_rnof2[0] * dels;
   t->ssnow_runoff[0] = t->ssnow_runoff[0] * dels;
+  /* sumcflux, src/offline/casa_cable.F90:482-560 (icycle = 0, kstart = 1),
+     called at src/offline/cable_serial.F90:713-715: running sums of the
+     carbon fluxes times the step length; it sets fnee to cbm's value again */
+  {
+    double pn = t->canopy_fpn[0] * dels, rd = t->canopy_frday[0] * dels;
+    double rpw = t->canopy_frpw[0] * dels, rpr = t->canopy_frpr[0] * dels;
+    double rp = t->canopy_frp[0] * dels, rs = t->canopy_frs[0] * dels;
+    if (ktau == 1) {
+      t->sum_flux_sumpn[0] = pn;
+      t->sum_flux_sumrd[0] = rd;
+      t->sum_flux_dsumpn[0] = pn;
+      t->sum_flux_dsumrd[0] = rd;
+      t->sum_flux_sumrpw[0] = rpw;
+      t->sum_flux_sumrpr[0] = rpr;
+      t->sum_flux_sumrp[0] = rp;
+      t->sum_flux_dsumrp[0] = rp;
+      t->sum_flux_sumrs[0] = rs;
+    } else {
+      t->sum_flux_sumpn[0] = t->sum_flux_sumpn[0] + pn;
+      t->sum_flux_sumrd[0] = t->sum_flux_sumrd[0] + rd;
+      t->sum_flux_dsumpn[0] = t->sum_flux_dsumpn[0] + pn;
+      t->sum_flux_dsumrd[0] = t->sum_flux_dsumrd[0] + rd;
+      t->sum_flux_sumrpw[0] = t->sum_flux_sumrpw[0] + rpw;
+      t->sum_flux_sumrpr[0] = t->sum_flux_sumrpr[0] + rpr;
+      t->sum_flux_sumrp[0] = t->sum_flux_sumrp[0] + rp;
+      t->sum_flux_dsumrp[0] = t->sum_flux_dsumrp[0] + rp;
+      t->sum_flux_sumrs[0] = t->sum_flux_sumrs[0] + rs;
+    }
+  }
   /* mass_balance, src/offline/cable_checks.F90:472-551 */
   {
     if (ktau == 1) t->drv_owb[0] = t->ssnow_wbtot[0];

@@ -7,8 +7,8 @@
 !
 ! It reproduces the offline serial driver's contract around the cbm call
 ! (src/offline/cable_serial.F90:437-442 before the time loop, :572-575 before
-! each call, :594-605 around it, :725-729 after it) and the cable.nml
-! switches of the shipped configuration (src/offline/cable.nml).
+! each call, :594-605 around it, :713-715 and :725-729 after it) and the
+! cable.nml switches of the shipped configuration (src/offline/cable.nml).
 !
 ! Protocol (stream files, little endian):
 !   input : int64 mp, then records  int32 op, ...
@@ -24,6 +24,9 @@ USE cable_common_module, ONLY: cable_user, cable_runtime, snmin, wiltParam, satu
 USE cable_cbm_module, ONLY: cbm
 USE cable_climate_type_mod, ONLY: climate_type
 USE cable_phys_constants_mod, ONLY: CSBOLTZ => SBOLTZ, CEMLEAF => EMLEAF, CEMSOIL => EMSOIL
+USE casadimension, ONLY: icycle
+USE casavariable, ONLY: casa_flux
+USE sumcflux_mod, ONLY: sumcflux
 USE ref_fields
 IMPLICIT NONE
 CHARACTER(LEN=512) :: inpath, outpath
@@ -35,6 +38,7 @@ REAL :: dels
 CHARACTER(LEN=40) :: name
 REAL(8), ALLOCATABLE :: buf(:)
 TYPE (climate_type) :: climate
+TYPE (casa_flux) :: casaflux   ! not referenced with icycle = 0
 REAL, ALLOCATABLE :: c1(:,:), rhoch(:,:), xk(:,:)
 INTEGER(8) :: clk0, clk1, clkrate
 REAL(8) :: step_seconds = 0.0_8
@@ -60,6 +64,7 @@ cable_user%consistency_check = .TRUE.
 snmin = 1.0
 wiltParam = 0.5
 satuParam = 0.8
+icycle = 0
 
 CALL alloc_cbm_var(air, mp)
 CALL alloc_cbm_var(bgc, mp)
@@ -117,6 +122,9 @@ DO
     ssnow%rnof1 = ssnow%rnof1*dels
     ssnow%rnof2 = ssnow%rnof2*dels
     ssnow%runoff = ssnow%runoff*dels
+    ! src/offline/cable_serial.F90:713-715 (kstart = 1)
+    CALL sumcflux(ktau4, 1, ktau4, dels, bgc, canopy, soil, ssnow, sum_flux, veg, &
+                  met, casaflux, .FALSE.)
     ! src/offline/cable_serial.F90:725-729 (check%mass_bal, check%energy_bal)
     CALL drv_mass_balance(dels, ktau4)
     CALL drv_energy_balance()
