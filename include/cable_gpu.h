@@ -119,10 +119,12 @@ int cable_gpu_cbm(cable_gpu_ctx *ctx, int ktau, double dels);
 
 /* One step of the offline driver: its contract before cbm (ofsd, oldcansto,
    vlai = 0 where iveg >= 14; src/offline/cable_serial.F90:572-575), cbm, the
-   unit conversions after it (:602-605) and the per-tile water and energy
-   balance checks (src/offline/cable_checks.F90:472-618, called at
+   unit conversions after it (:602-605), the running carbon sums of sumcflux
+   (src/offline/casa_cable.F90:482-560, called at :713-715; the sum_flux%*
+   fields, restarted at ktau = 1) and the per-tile water and energy balance
+   checks (src/offline/cable_checks.F90:472-618, called at
    cable_serial.F90:725-729; the driver's saved previous water store is the
-   field drv%owb). */
+   field drv%owb).  cable_gpu_run performs the same driver step. */
 int cable_gpu_step(cable_gpu_ctx *ctx, int ktau, double dels);
 
 /* ---- device-resident run (production / benchmark path) ---------------- */
