@@ -1167,7 +1167,9 @@ struct Stepper {
   }
 
   // ===================================================================
-  // soil and snow; REAL(r_2) quantities and both tridiagonal solves in FP64
+  // soil and snow; the REAL(r_2) soil water and heat state, its updates and
+  // both tridiagonal solves in FP64 (the conductivities' elementary functions
+  // in the working precision)
 
   // src/science/soilsnow/cbl_trimb.F90:17-53
   template <int KMAX>

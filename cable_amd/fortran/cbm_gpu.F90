@@ -17,7 +17,8 @@
 ! Device precision follows the host build unless cable_gpu_precision is set
 ! before the first call: a build with 8-byte default REAL (-fdefault-real-8)
 ! gets the FP64 path; the shipped build (default REAL FP32, REAL(r_2) FP64)
-! gets the mixed path, which keeps exactly that split on the device.  FP64
+! gets the mixed path, which stores the fields with that split on the device
+! and computes the canopy in FP32, the soil state and solves in FP64.  FP64
 ! arithmetic on FP32-stored fields is not offered: the reference compares
 ! some carried values exactly (e.g. ssnow%cls against 1.1335 in
 ! src/science/soilsnow/cbl_snowAccum.F90), and those tests flip when an FP64
