@@ -373,25 +373,29 @@ struct Stepper {
   }
 
   // src/science/canopy/cbl_friction_vel.F90:112-166
+  // psim and psis: the reference blends the stable and unstable forms as
+  // z*stable + (1-z)*unstable with z = 0.5 + SIGN(0.5, zeta), which is 0 or 1;
+  // both forms are finite for the clamped zeta, so the blend returns the
+  // selected form exactly and only that form is evaluated
   CBM_HD static R psim(R zeta) {
     const R gu = K(16.0), a_ = K(1.0), b = K(0.667), xc = K(5.0), d = K(0.35);
     R z = K(0.5) + SIGN(K(0.5), zeta);
-    R stable = -a_ * zeta - b * (zeta - xc / d) * exp(-d * zeta) - b * xc / d;
-    R x = pow(K(1.0) + gu * fabs(zeta), K(0.25));
-    R unstable = log((K(1.0) + x * x) * p2(K(1.0) + x) / K(8)) - K(2.0) * atan(x) + PI_C * K(0.5);
-    return z * stable + (K(1.0) - z) * unstable;
+    if (z > K(0.5)) return -a_ * zeta - b * (zeta - xc / d) * exp(-d * zeta) - b * xc / d;
+    R x = root4(K(1.0) + gu * fabs(zeta));
+    return log((K(1.0) + x * x) * p2(K(1.0) + x) / K(8)) - K(2.0) * atan(x) + PI_C * K(0.5);
   }
 
   // src/science/canopy/cbl_friction_vel.F90:168-221
   CBM_HD static R psis(R zeta) {
     const R gu = K(16.0), a_ = K(1.0), b = K(0.667), c = K(5.0), d = K(0.35);
     R z = K(0.5) + SIGN(K(0.5), zeta);
-    R stzeta = MAX(K(0.), zeta);
-    R stable = -pow(K(1.) + K(2.) / K(3.) * a_ * stzeta, K(3.) / K(2.)) -
-               b * (stzeta - c / d) * exp(-d * stzeta) - b * c / d + K(1.);
+    if (z > K(0.5)) {
+      R stzeta = MAX(K(0.), zeta);
+      return -pow1p5(K(1.) + K(2.) / K(3.) * a_ * stzeta) - b * (stzeta - c / d) * exp(-d * stzeta) - b * c / d +
+             K(1.);
+    }
     R y = sqrt(K(1.0) + gu * fabs(zeta));
-    R unstable = K(2.0) * log((K(1) + y) * K(0.5));
-    return z * stable + (K(1.0) - z) * unstable;
+    return K(2.0) * log((K(1) + y) * K(0.5));
   }
 
   // src/science/canopy/cbl_dryLeaf.F90:779-877
@@ -583,7 +587,7 @@ struct Stepper {
         R dleaf = PV(veg_dleaf, 0);
         c.ghwet = K(2.0) * sum_gbh;
         R gras = MAX(K(1.0e-6), K(1.595E8) * fabs(tlfx - tvair) * p3(dleaf));
-        R g25 = pow(gras, K(0.25));
+        R g25 = root4(gras);
         c.gbhf[0] = fvl0 * t.air_cmolar[0] * K(0.5) * DHEAT * (g25) / dleaf;
         c.gbhf[1] = fvl1 * t.air_cmolar[0] * K(0.5) * DHEAT * (g25) / dleaf;
         c.gbhf[0] = MAX(K(1.e-6), c.gbhf[0]);
@@ -998,7 +1002,7 @@ struct Stepper {
         t.rad_lwabv[0] = CAPP * RMAIR * (c.tlfy - t.met_tk[0]) * c.sum_rad_gradis;
         R q = t.rad_lwabv[0] / (K(2.0) * (K(1.0) - transd) * SBOLTZ * EMLEAF) + p4(tvrad);
         if (q > K(0.0))
-          t.canopy_tv[0] = pow(q, K(0.25));
+          t.canopy_tv[0] = root4(q);
         else
           t.canopy_tv[0] = tvrad;
       } else {
@@ -2017,7 +2021,7 @@ struct Stepper {
     t.canopy_fe[0] = (R)((double)t.canopy_fev[0] + t.canopy_fes[0]);
     t.canopy_rnet[0] = t.canopy_fns[0] + t.canopy_fnv[0];
     R transd = t.rad_transd[0];
-    t.rad_trad[0] = pow((K(1.) - transd) * p4(t.canopy_tv[0]) + transd * p4(t.ssnow_tss[0]), K(0.25));
+    t.rad_trad[0] = root4((K(1.) - transd) * p4(t.canopy_tv[0]) + transd * p4(t.ssnow_tss[0]));
     carbon();
     R fpn = t.canopy_fpn[0];
     t.canopy_fnpp[0] = -K(1.0) * fpn - t.canopy_frp[0];
