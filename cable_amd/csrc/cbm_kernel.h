@@ -480,12 +480,18 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
   const int64_t nblk = (a.mp + TILES - 1) / TILES;
   for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
     const int64_t t0 = blk * TILES;
-    // a block whose tiles carry no leaves has nothing to predict (the warm
-    // start applies to vegetated tiles only).  The vote is also the barrier
-    // between the previous block's layers and this block's feature rows.
+    // a block with no leaf-bearing tile in daylight has nothing to predict:
+    // the warm start applies to sunlit vegetated tiles only, and the test is
+    // the mixed step's own (both shortwave bands, fsd(:,2) = fsd(:,1), above
+    // RAD_THRESH in FP32).  The vote is also the barrier between the previous
+    // block's layers and this block's feature rows.
     {
       const int64_t t = t0 + threadIdx.x;
-      const bool leafy = threadIdx.x < TILES && t < a.mp && ld<int>(a.F[CABLE_F_veg_iveg], t) < 14;
+      bool leafy = false;
+      if (threadIdx.x < TILES && t < a.mp && ld<int>(a.F[CABLE_F_veg_iveg], t) < 14) {
+        const float fsd = ld<float>(a.F[CABLE_F_met_fsd], t + a.foff);
+        leafy = fsd + fsd > 0.001f;
+      }
       if (!__syncthreads_or(leafy) && !xraw) continue;
     }
     // features of tile threadIdx.x, normalised, as BF16 row threadIdx.x

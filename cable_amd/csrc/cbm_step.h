@@ -918,9 +918,10 @@ struct Stepper {
       // warm start of the leaf fixed point of the first NITER pass at the
       // surrogate's prediction of where that pass converges: leaf
       // temperature, the leaf-surface CO2 of both leaf classes, and the leaf
-      // vapour-pressure deficit that temperature implies (without
-      // predictions the iteration starts as the reference's does)
-      if (veg && a.pred) {
+      // vapour-pressure deficit that temperature implies.  The network is
+      // evaluated for sunlit leaf-bearing tiles only; without light (and
+      // without predictions) the iteration starts as the reference's does
+      if (sunlit_veg && a.pred) {
         c.tlfx = t.met_tk[0] + (R)pred[0];
         c.csx[0] = MAX(K(1.0e-4), t.met_ca[0] * (R)pred[1]);
         c.csx[1] = MAX(K(1.0e-4), t.met_ca[0] * (R)pred[2]);

@@ -67,19 +67,21 @@ def test_uncapped_hybrid_without_network_is_the_plain_step():
 def test_network_warm_start_cuts_fallbacks_and_passes():
     """Train on samples of one workload, run another: at the same pass cap the
     network's starting point leaves fewer tiles to the fallback, and with no
-    cap it needs fewer leaf passes, than the reference's own start."""
+    cap it needs fewer leaf passes, than the reference's own start.  The
+    network warm-starts sunlit leaf-bearing tiles only, so the comparison
+    runs the daylight steps 12..35 (06:00 to 17:30)."""
     from cable_amd import surrogate as S
     print("[hybrid] network gain", flush=True)
     params, norm, info = S.train_on_gpu(n_tiles=65536, nsteps=48, epochs=2.0,
                                         log=lambda m: print(m, flush=True))
     assert info["loss_last"] < 0.2 * info["loss_first"]
-    wl = Workload(65536, 24)
+    wl = Workload(65536, 36)
     res = {}
     for use in (True, False):
         for cap in (3, 20):
             g = _ctx(wl)
             g.set_surrogate(params, norm)
-            g.run_hybrid(0, 24, wl.dels, max_passes=cap, resid_max=0.5, use_surrogate=use)
+            g.run_hybrid(12, 24, wl.dels, max_passes=cap, resid_max=0.5, use_surrogate=use)
             fb, ts = g.hybrid_counters()
             res[(use, cap)] = (fb / ts, g.dryleaf_passes())
             g.close()
