@@ -266,9 +266,10 @@ int cable_gpu_copy_samples(cable_gpu_ctx *ctx, int64_t first, int64_t n, float *
    include/cable_mlp.h) */
 int cable_gpu_set_surrogate(cable_gpu_ctx *ctx, const float *params, const float *norm);
 /* Run steps k0..k0+nsteps-1 with the hybrid method: per step the network
-   predicts each tile's leaf state, the leaf fixed point starts there and
-   runs at most max_passes passes per NITER pass (the reference runs up to
-   20), and every tile whose leaf energy-balance residual beyond the
+   predicts the leaf state of each sunlit leaf-bearing tile, the leaf fixed
+   point starts there (elsewhere at the reference's own start) and runs at
+   most max_passes passes per NITER pass (the reference runs up to 20), and
+   every tile whose leaf energy-balance residual beyond the
    reference's 0.1 K tolerance exceeds resid_max (W/m2) is stepped again
    with the plain method from its unchanged state.  use_surrogate = 0 runs
    the same capped method from the reference's own starting point (the
@@ -279,9 +280,9 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, in
 int cable_gpu_hybrid_counters(cable_gpu_ctx *ctx, int64_t *fallback_tile_steps, int64_t *tile_steps);
 /* The surrogate's view of step k from the current state, as the hybrid step
    computes it: x (mp x 24) the raw features, pred (mp x 8) the denormalised
-   predictions, host arrays in the caller's tile order.  Predictions of tiles
-   in 256-tile blocks without a leaf-bearing vegetation type are computed
-   too. */
+   predictions, host arrays in the caller's tile order.  Predictions are
+   computed for every tile, including those the hybrid step does not
+   warm-start (no leaves, or no daylight). */
 int cable_gpu_surrogate_eval(cable_gpu_ctx *ctx, int k, float *x, float *pred);
 
 const char *cable_gpu_last_error(void);
