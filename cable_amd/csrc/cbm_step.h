@@ -25,6 +25,8 @@ struct Stepper {
   float tgt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   float resid = 0.0f;
   bool vegetated = false;
+  bool capped = false;         // MODE_HYBRID: a sunlit leaf-bearing tile, whose
+                               // leaf passes are capped and judged
   R leaf_temperature = R(0);   // the leaf fixed point's final temperature
 
   // ---- constants: src/params/*, src/util/cable_common.F90:217-222 ----
@@ -574,8 +576,9 @@ struct Stepper {
     const R fvl0 = t.rad_fvlai[0], fvl1 = t.rad_fvlai[1];
     const bool leaves = fvl0 + fvl1 > LAI_THRESH;
     // the reference's fixed count of 20 passes; the hybrid polishes the
-    // surrogate's start with at most a.max_passes
-    const int kmax = (MODE == MODE_HYBRID) ? a.max_passes : 20;
+    // surrogate's start of a sunlit leaf-bearing tile with at most
+    // a.max_passes (every other tile runs the reference's iteration)
+    const int kmax = (MODE == MODE_HYBRID && capped) ? a.max_passes : 20;
     for (int k = 1; k <= kmax; k++) {
       bool active = (vlaiw > LAI_THRESH && abs_deltlf > K(0.1));
       // a tile that has converged keeps every value: the remaining passes of
@@ -704,8 +707,9 @@ struct Stepper {
       // leaf energy-balance residual at the last pass's starting temperature:
       // the leaf sensible-plus-radiative conductance times the temperature
       // change that pass still made (cr * sum(ghr) * |dT|), counted beyond
-      // the 0.1 K the reference's own convergence test accepts
-      if (vlaiw > LAI_THRESH && abs_deltlf > K(0.1)) {
+      // the 0.1 K the reference's own convergence test accepts (capped tiles
+      // only: the others ran the reference's iteration and never fall back)
+      if (capped && vlaiw > LAI_THRESH && abs_deltlf > K(0.1)) {
         const float r = (float)(CAPP * RMAIR * (ghr[0] + ghr[1]) * (abs_deltlf - K(0.1)));
         resid = r > resid ? r : resid;
       }
@@ -920,7 +924,10 @@ struct Stepper {
       // temperature, the leaf-surface CO2 of both leaf classes, and the leaf
       // vapour-pressure deficit that temperature implies.  The network is
       // evaluated for sunlit leaf-bearing tiles only; without light (and
-      // without predictions) the iteration starts as the reference's does
+      // without predictions) the iteration starts as the reference's does.
+      // Those tiles' passes are capped with or without a network, so a run
+      // without one is the same capped method from the reference's start
+      capped = sunlit_veg;
       if (sunlit_veg && a.pred) {
         c.tlfx = t.met_tk[0] + (R)pred[0];
         c.csx[0] = MAX(K(1.0e-4), t.met_ca[0] * (R)pred[1]);
