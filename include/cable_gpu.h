@@ -267,9 +267,10 @@ int cable_gpu_copy_samples(cable_gpu_ctx *ctx, int64_t first, int64_t n, float *
 int cable_gpu_set_surrogate(cable_gpu_ctx *ctx, const float *params, const float *norm);
 /* Run steps k0..k0+nsteps-1 with the hybrid method: per step the network
    predicts the leaf state of each sunlit leaf-bearing tile, the leaf fixed
-   point starts there (elsewhere at the reference's own start) and runs at
-   most max_passes passes per NITER pass (the reference runs up to 20), and
-   every tile whose leaf energy-balance residual beyond the
+   point of those tiles starts there and runs at most max_passes passes per
+   NITER pass (the reference runs up to 20; every other tile runs the
+   reference's iteration), and every capped tile whose leaf energy-balance
+   residual beyond the
    reference's 0.1 K tolerance exceeds resid_max (W/m2) is stepped again
    with the plain method from its unchanged state.  use_surrogate = 0 runs
    the same capped method from the reference's own starting point (the
