@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per tile-step: prognostic state read and written once,
 # 8 forcing fields and the per-tile parameters read once, 8 outputs written
 ALGO_BYTES = {"fp64": 1016, "mixed": 756, "hybrid": 764}
-CPU_SAMPLE_TILES = 16384
+CPU_SAMPLE_TILES = 49152
 
 
 HEADLINE = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes", "canopy%fgpp",
