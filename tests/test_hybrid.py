@@ -64,6 +64,31 @@ def test_uncapped_hybrid_without_network_is_the_plain_step():
         np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-2, err_msg=k)
 
 
+def test_night_steps_run_the_reference_iteration():
+    """Without daylight no tile is warm-started or capped: with a network and
+    a cap of one pass, the night steps (00:00 to 05:30) queue nothing for the
+    fallback and give the plain step's fluxes."""
+    from cable_amd.surrogate import NIN, NOUT
+    from oracle import mlp_oracle as O
+    print("[hybrid] night steps", flush=True)
+    wl = Workload(8192, 12)
+    p = O.init_params(5).astype(np.float32)
+    unit = np.concatenate([np.zeros(NIN), np.ones(NIN), np.zeros(NOUT), np.ones(NOUT)]).astype(np.float32)
+    g = _ctx(wl)
+    g.set_surrogate(p, unit)
+    g.run_hybrid(0, 12, wl.dels, max_passes=1, resid_max=0.5, use_surrogate=True)
+    fb, ts = g.hybrid_counters()
+    a = _outputs(g)
+    g.close()
+    g = _ctx(wl)
+    g.run(0, 12, wl.dels)
+    b = _outputs(g)
+    g.close()
+    assert fb == 0 and ts == 8192 * 12
+    for k in a:
+        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-2, err_msg=k)
+
+
 def test_network_warm_start_cuts_fallbacks_and_passes():
     """Train on samples of one workload, run another: at the same pass cap the
     network's starting point leaves fewer tiles to the fallback, and with no
