@@ -225,6 +225,10 @@ def _cpu_reference_more(wl, steps, log):
 
 
 TRAIN = {"n_tiles": 262144, "nsteps": 96, "epochs": 2.0, "max_samples": 16777216}
+# the hybrid's polish passes per NITER pass and its leaf energy residual limit
+# (W/m2).  Over the week-long run a cap of 3 was more accurate but slower
+# than 4 (5.5% of tile-steps fall back instead of 0.4%)
+HYB_CAP, HYB_RESID = 4, 0.5
 
 
 def production(args, log):
@@ -265,7 +269,7 @@ def production(args, log):
                 g.sync()
                 t = time.perf_counter()
                 if name == "hybrid":
-                    g.run_hybrid(p0, pn, wl.dels, 4, 0.5)
+                    g.run_hybrid(p0, pn, wl.dels, HYB_CAP, HYB_RESID)
                 else:
                     g.run(p0, pn, wl.dels)
                 g.sync()
@@ -292,7 +296,7 @@ def production(args, log):
         forms[f"mixed_{n}"]["flux_error_vs_fp64_noon_last_day"] = flux_errors(noon["mixed"], noon["fp64"])
         forms[f"hybrid_{n}"]["flux_error_vs_fp64_noon_last_day"] = flux_errors(noon["hybrid"], noon["fp64"])
     forms[f"hybrid_{n}"]["fallback_fraction"] = fb / ts
-    forms[f"hybrid_{n}"]["max_passes"], forms[f"hybrid_{n}"]["resid_max_Wm2"] = 4, 0.5
+    forms[f"hybrid_{n}"]["max_passes"], forms[f"hybrid_{n}"]["resid_max_Wm2"] = HYB_CAP, HYB_RESID
     forms[f"hybrid_{n}"]["surrogate_training"] = info
     forms["hybrid_vs_fp64"] = wall["fp64"] / wall["hybrid"]
     forms["mixed_vs_fp64"] = wall["fp64"] / wall["mixed"]
@@ -375,12 +379,12 @@ def main():
         flops = 3 * 2 * (24 * 128 + 2 * 128 * 128 + 128 * 8) * info["batches"] * 2048
         info["training_TFLOPs"] = flops / info["train_s"] / 1e12
         hyb = run_form(1048576, args.steps, args.warmup, "mixed", log, co, keep=True,
-                       hybrid={"params": params, "norm": norm, "max_passes": 4, "resid_max": 0.5})
+                       hybrid={"params": params, "norm": norm, "max_passes": HYB_CAP, "resid_max": HYB_RESID})
         ref = f64.pop("_out")
         mix["flux_error_vs_fp64_after_run"] = flux_errors(mix.pop("_out"), ref)
         hyb["flux_error_vs_fp64_after_run"] = flux_errors(hyb.pop("_out"), ref)
         hyb["surrogate_training"] = info
-        hyb["max_passes"], hyb["resid_max_Wm2"] = 4, 0.5
+        hyb["max_passes"], hyb["resid_max_Wm2"] = HYB_CAP, HYB_RESID
         forms["dropin_fp64_262144"] = dropin_form(262144, 3, log)
         forms["fp64_1048576"] = f64
         forms["mixed_1048576"] = mix
