@@ -89,6 +89,30 @@ def test_night_steps_run_the_reference_iteration():
         np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-2, err_msg=k)
 
 
+def test_hybrid_is_deterministic():
+    """The fallback queue is filled by atomics in whatever order the waves
+    reach it; each tile's result does not depend on that order.  Two runs of
+    the same daylight steps, with a cap that sends tiles to the fallback,
+    give the same bits and the same counters."""
+    from cable_amd.surrogate import NIN, NOUT
+    from oracle import mlp_oracle as O
+    print("[hybrid] determinism", flush=True)
+    wl = Workload(8192, 24)
+    p = O.init_params(9).astype(np.float32)
+    unit = np.concatenate([np.zeros(NIN), np.ones(NIN), np.zeros(NOUT), np.ones(NOUT)]).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        g = _ctx(wl)
+        g.set_surrogate(p, unit)
+        g.run_hybrid(14, 10, wl.dels, max_passes=2, resid_max=0.5, use_surrogate=True)
+        runs.append((g.hybrid_counters(), _outputs(g)))
+        g.close()
+    (c0, a), (c1, b) = runs
+    assert c0 == c1 and c0[0] > 0
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_network_warm_start_cuts_fallbacks_and_passes():
     """Train on samples of one workload, run another: at the same pass cap the
     network's starting point leaves fewer tiles to the fallback, and with no
