@@ -223,8 +223,11 @@ def train_on_gpu(n_tiles=262144, nsteps=64, epochs=2.0, batch=2048, lr=2e-3, see
 
 
 def main(argv=None):
-    """python -m cable_amd.surrogate [--tiles N] [--steps K] [--epochs E] [--out F.npz]:
-    capture samples and train on the GPU; prints the training summary as JSON."""
+    """python -m cable_amd.surrogate [--tiles N] [--steps K] [--epochs E] [--out F.npz]
+    [--hybrid TILES K0 NSTEPS]: capture samples and train on the GPU, then
+    optionally run the hybrid step with the trained network on another
+    workload of TILES tiles, steps K0..K0+NSTEPS-1 (daylight steps exercise
+    the inference); prints the training summary as JSON."""
     import argparse
     import json
     import sys
@@ -234,11 +237,26 @@ def main(argv=None):
     ap.add_argument("--epochs", type=float, default=2.0)
     ap.add_argument("--max-samples", type=int, default=None)
     ap.add_argument("--out", default=None, help="save params and norm (numpy .npz)")
+    ap.add_argument("--hybrid", type=int, nargs=3, metavar=("TILES", "K0", "NSTEPS"), default=None)
     args = ap.parse_args(argv)
     params, norm, info = train_on_gpu(args.tiles, args.steps, args.epochs, max_samples=args.max_samples,
                                       log=lambda m: print(m, file=sys.stderr, flush=True))
     if args.out:
         np.savez(args.out, params=params, norm=norm)
+    if args.hybrid:
+        from . import gpu
+        from .workload import DEFAULT_SEED, Workload
+        n, k0, ns = args.hybrid
+        wl = Workload(n, k0 + ns, seed=DEFAULT_SEED + 2)
+        g = gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False,
+                              order=gpu.class_order(wl.iveg, wl.isoilm, wl.vlai))
+        g.set_surrogate(params, norm)
+        g.run_hybrid(k0, ns, wl.dels, 4, 0.5)
+        g.sync()
+        fb, ts = g.hybrid_counters()
+        info["hybrid"] = {"tiles": n, "first_step": k0, "steps": ns, "device_ms": g.last_run_ms(),
+                          "fallback_fraction": fb / ts}
+        g.close()
     print(json.dumps(info), flush=True)
 
 
