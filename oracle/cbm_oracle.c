@@ -32,6 +32,37 @@
 #define MS 6
 #define MSN 3
 
+/* Test hook for measuring the reference's own sensitivity: an offset (K)
+   added to the leaf temperature the canopy's fixed point starts from
+   (tlfx = tlfy = met%tk, src/science/canopy/cable_canopy.F90:213-214).  It
+   is 0 unless oracle_set_leaf_start_offset sets it; x + 0.0 == x, so the
+   default leaves every result bit-identical. */
+static double leaf_start_offset = 0.0;
+void oracle_set_leaf_start_offset(double dk) { leaf_start_offset = dk; }
+
+/* Test hook for measuring how far one-ulp differences of the elementary
+   functions carry: with a shift of +1 (-1) every exp, exp2, log, pow and
+   atan result is moved one ulp up (down).  A device math library rounds
+   these functions differently from the host's by about an ulp, so the
+   oracle's own response to the shift bounds the part of a device-host
+   difference that the computation, not the device code, produces.  0 (the
+   default) returns the library's value unchanged. */
+static int ulp_shift = 0;
+void oracle_set_ulp_shift(int s) { ulp_shift = s; }
+static inline double shifted(double y) {
+  return ulp_shift == 0 ? y : nextafter(y, ulp_shift > 0 ? INFINITY : -INFINITY);
+}
+static inline double o_exp(double x) { return shifted(exp(x)); }
+static inline double o_exp2(double x) { return shifted(exp2(x)); }
+static inline double o_log(double x) { return shifted(log(x)); }
+static inline double o_pow(double x, double y) { return shifted(pow(x, y)); }
+static inline double o_atan(double x) { return shifted(atan(x)); }
+#define exp(x) o_exp(x)
+#define exp2(x) o_exp2(x)
+#define log(x) o_log(x)
+#define pow(x, y) o_pow(x, y)
+#define atan(x) o_atan(x)
+
 /* ---- constants: src/params/ (phys, photo, other, maths) and src/util/cable_common.F90 ---- */
 static const double TFRZ = 273.16, SBOLTZ = 5.67e-8, EMSOIL = 1.0,
                     EMLEAF = 1.0, CAPP = 1004.64, HL = 2.5014e6,
@@ -954,8 +985,8 @@ static void define_canopy(Tile *t, const Globals *g, double dels, int sunlit_veg
   t->met_dva[0] = (qstvair - t->met_qvair[0]) * RMAIR / RMH2O * t->met_pmb[0] * 100.0;
   c->dsx = t->met_dva[0];
   c->dsx = MAX(c->dsx, 0.0);
-  c->tlfx = t->met_tk[0];
-  c->tlfy = t->met_tk[0];
+  c->tlfx = t->met_tk[0] + leaf_start_offset;
+  c->tlfy = t->met_tk[0] + leaf_start_offset;
   double ortsoil = t->ssnow_rtsoil[0];
   int isflag = (int)t->ssnow_isflag[0];
   t->ssnow_tss[0] = (double)(1 - isflag) * t->ssnow_tgg[0] + (double)isflag * t->ssnow_tggsn[0];

@@ -26,6 +26,10 @@ def lib():
         _lib.oracle_cbm_step.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
                                          ctypes.POINTER(ctypes.c_double)]
+        _lib.oracle_set_ulp_shift.restype = None
+        _lib.oracle_set_ulp_shift.argtypes = [ctypes.c_int]
+        _lib.oracle_set_leaf_start_offset.restype = None
+        _lib.oracle_set_leaf_start_offset.argtypes = [ctypes.c_double]
         assert _lib.oracle_nfields() == NFIELDS
     return _lib
 
@@ -65,6 +69,32 @@ class OracleRun:
         self.passes += lib().oracle_cbm_step(
             self.mp, self.ktau, float(self.wl.dels), 1 if self.ktau == 1 else 0, ptrs,
             self.g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def step_sensitivity(self, forcing_k: dict, names):
+        """One step, as step(), and its response to a one-ulp shift of the
+        elementary functions (oracle_set_ulp_shift): returns, per field of
+        ``names``, max(|o+ - o|, |o- - o|) per value, where o+ and o- are
+        the step's results with every exp, exp2, log, pow and atan result
+        moved one ulp up and down, from the same starting state."""
+        pre = {k: v.copy() for k, v in self.f.items()}
+        ktau, passes = self.ktau, self.passes
+        shifted = []
+        for sh in (1, -1):
+            for k, v in pre.items():
+                self.f[k][...] = v
+            self.ktau = ktau
+            lib().oracle_set_ulp_shift(sh)
+            try:
+                self.step(forcing_k)
+            finally:
+                lib().oracle_set_ulp_shift(0)
+            shifted.append({k: self.f[k].copy() for k in names})
+        for k, v in pre.items():
+            self.f[k][...] = v
+        self.ktau, self.passes = ktau, passes
+        self.step(forcing_k)
+        return {k: np.maximum(np.abs(shifted[0][k] - self.f[k]), np.abs(shifted[1][k] - self.f[k])).reshape(-1)
+                for k in names}
 
     def run(self, nsteps=None, outputs=(), every_step=True):
         nsteps = self.wl.nsteps if nsteps is None else nsteps
