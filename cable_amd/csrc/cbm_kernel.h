@@ -32,11 +32,11 @@ enum { ROLE_PARAM = 0, ROLE_FORC = 1, ROLE_STATE = 2, ROLE_DIAG = 3 };
 template <class T>
 using gptr = __attribute__((address_space(1))) T *;
 template <class T>
-__device__ __forceinline__ T ld(const void *p, int64_t i) {
+CBM_HD T ld(const void *p, int64_t i) {
   return ((const gptr<const T>)p)[i];
 }
 template <class T, class V>
-__device__ __forceinline__ void st(void *p, int64_t i, V v) {
+CBM_HD void st(void *p, int64_t i, V v) {
   ((gptr<T>)p)[i] = static_cast<T>(v);
 }
 
@@ -58,8 +58,7 @@ constexpr int role_id(int r) { return r; }
 // class-table rows.  The capture step and the inference kernel both call
 // this, so training and use see the same features.
 template <class R>
-__device__ inline void surrogate_features(const StepArgs &a, int64_t t, const double *vrow, const double *srow,
-                                          float *x) {
+CBM_HD void surrogate_features(const StepArgs &a, int64_t t, const double *vrow, const double *srow, float *x) {
   using DT = typename DevT<R, 'F'>::type;
   void *const *F = a.F;
   const int64_t mp = a.mp, f = t + a.foff;
@@ -122,6 +121,239 @@ __device__ inline unsigned wave_append32(bool take, unsigned *count) {
   return base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// what a tile's step reports to the kernel around it
+template <int MODE, bool CONS>
+struct TileOut {
+  int passes = 0;                      // dryLeaf passes (work counter)
+  bool fallback = false;               // MODE_HYBRID: the tile must be stepped again
+  bool emit = false;                   // MODE_CAPTURE: the tile yields a sample
+  double cons[CONS ? 4 : 1] = {0.0};   // CONS: the tile's wbal, ebal, fpn, fe
+  float feat[MODE == MODE_CAPTURE ? mlp::NIN : 1];
+  float tgt[MODE == MODE_CAPTURE ? mlp::NOUT : 1];
+};
+
+// One tile's step: gather its fields from the arrays of a.F (FULL: every
+// field; otherwise the resident run's compact set), run the physics, scatter
+// the results.  vtab/stab are the class tables (TABLES), staged in LDS by the
+// kernel.  Host and device code: the test suite also runs it on the CPU, on
+// host arrays, under the sanitizers.
+template <class R, bool FULL, bool TABLES, int MODE, bool CONS>
+CBM_HD void step_tile(const StepArgs &a, int64_t t, const double *vtab, const double *stab,
+                      TileOut<MODE, CONS> &o) {
+  const int64_t mp = a.mp;
+  void *const *F = a.F;
+  Tile<R> T;
+  // ---- gather ----
+  if constexpr (FULL) {
+#define PARAM ROLE_PARAM
+#define FORC ROLE_FORC
+#define STATE ROLE_STATE
+#define DIAG ROLE_DIAG
+#define X(id, name, prec, n, role)                                                    \
+  {                                                                                   \
+    using DT = typename DevT<R, #prec[0]>::type;                                      \
+    const int64_t off = (role == ROLE_FORC) ? a.foff : 0;                             \
+    _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] =                           \
+        (decltype(T.id[0] + 0))ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t + off);     \
+  }
+    CABLE_FIELDS(X)
+#undef X
+#undef PARAM
+#undef FORC
+#undef STATE
+#undef DIAG
+  } else {
+    // prognostic state
+#define LDS_(id, n)                                                                   \
+  {                                                                                   \
+    using DT = typename DevT<R, 'F'>::type;                                           \
+    _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t); \
+  }
+#define LDD_(id, n)                                                                   \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<double>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+    T.ssnow_isflag[0] = ld<int>(F[CABLE_F_ssnow_isflag], t);
+    LDS_(ssnow_pudsto, 1) LDS_(ssnow_rtsoil, 1) LDS_(ssnow_wbtot1, 1) LDS_(ssnow_wbtot2, 1)
+    LDS_(ssnow_wb_lake, 1) LDS_(ssnow_owetfac, 1) LDS_(ssnow_snage, 1) LDS_(ssnow_snowd, 1)
+    LDS_(ssnow_ssdnn, 1) LDS_(ssnow_tss, 1)
+    LDS_(ssnow_tgg, 1) LDS_(ssnow_tggsn, 1) LDS_(canopy_cansto, 1)
+    LDD_(ssnow_wb, 1) LDD_(ssnow_wbice, 1)
+    // the canopy reads only the top soil and snow layer; the deeper layers,
+    // snow layers, carbon pools, gammzz(:,1) and the balance accumulators
+    // are loaded after the canopy (Stepper::run's `late`).  Soil liquid
+    // water as the previous step left it (wb - wbice), before this step's
+    // lake fix-up:
+#pragma unroll
+    for (int k = 0; k < MS; k++)
+      T.ssnow_wbliq[k] = ld<double>(F[CABLE_F_ssnow_wb], (int64_t)k * mp + t) -
+                         ld<double>(F[CABLE_F_ssnow_wbice], (int64_t)k * mp + t);
+    // per-tile parameters
+    T.veg_iveg[0] = ld<int>(F[CABLE_F_veg_iveg], t);
+    T.soil_isoilm[0] = ld<int>(F[CABLE_F_soil_isoilm], t);
+    LDS_(veg_vlai, 1)
+    T.soil_albsoil[0] = ld<typename DevT<R, 'F'>::type>(F[CABLE_F_soil_albsoil], t);
+    // compact forcing (offline reader's derivations)
+    {
+      using DT = typename DevT<R, 'F'>::type;
+      const int64_t i = t + a.foff;
+      R fsd = ld<DT>(F[CABLE_F_met_fsd], i);
+      T.met_fsd[0] = fsd;
+      T.met_fsd[1] = fsd;
+      R tk = ld<DT>(F[CABLE_F_met_tk], i);
+      T.met_tk[0] = tk;
+      T.met_tvair[0] = tk;
+      T.met_tvrad[0] = tk;
+      T.met_fld[0] = ld<DT>(F[CABLE_F_met_fld], i);
+      T.met_qv[0] = ld<DT>(F[CABLE_F_met_qv], i);
+      T.met_ua[0] = ld<DT>(F[CABLE_F_met_ua], i);
+      T.met_pmb[0] = ld<DT>(F[CABLE_F_met_pmb], i);
+      R pr = ld<DT>(F[CABLE_F_met_precip], i);
+      T.met_precip[0] = pr;
+      T.met_precip_sn[0] = (tk <= R(273.16)) ? pr : R(0.0);
+      T.met_coszen[0] = ld<DT>(F[CABLE_F_met_coszen], i);
+      T.met_ca[0] = (R)a.ca;
+      T.met_doy[0] = (R)a.doy;
+    }
+#undef LDS_
+#undef LDD_
+  }
+  const int iv = T.veg_iveg[0] - 1, is = T.soil_isoilm[0] - 1;
+  const double *vrow = TABLES ? vtab + (iv < 0 ? 0 : iv) * VC_N : nullptr;
+  const double *srow = TABLES ? stab + (is < 0 ? 0 : is) * SC_N : nullptr;
+  Stepper<R, FULL, TABLES, MODE> S(T, a, vrow, srow);
+  auto late = [&]() {
+    if constexpr (!FULL) {
+      void *const *F = a.F;
+      using DT = typename DevT<R, 'F'>::type;
+#define LDS_(id, n) \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+#define LDD_(id, n) \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<double>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+      // a fresh read: nothing has written these since the gather, and the
+      // compiler must not keep the gather's values live to reuse them
+      __asm__ __volatile__("" ::: "memory");
+#define LDL_(T_, id, k0, n) \
+  { _Pragma("unroll") for (int k = k0; k < n; k++) T.id[k] = ld<T_>(F[CABLE_F_##id], (int64_t)k * mp + t); }
+      LDL_(DT, ssnow_tgg, 1, 6) LDL_(DT, ssnow_tggsn, 1, 3) LDL_(double, ssnow_wb, 1, 6)
+      LDL_(double, ssnow_wbice, 1, 6)
+#undef LDL_
+      // values the canopy reads only before its stability iterations and
+      // soil, snow and the balances read again: the shortwave forcing, the
+      // precipitation and its snow part, and the step's starting surface
+      // temperature (otss, which the step sets from the stored tss).  FP64
+      // only: the FP32 kernel's register allocation spills more with it
+      if constexpr (sizeof(R) == 8) {
+        const int64_t i = t + a.foff;
+        const R fsd = ld<DT>(F[CABLE_F_met_fsd], i), pr = ld<DT>(F[CABLE_F_met_precip], i);
+        T.met_fsd[0] = fsd;
+        T.met_fsd[1] = fsd;
+        T.met_precip[0] = pr;
+        T.met_precip_sn[0] = (T.met_tk[0] <= R(273.16)) ? pr : R(0.0);
+        T.ssnow_otss[0] = ld<DT>(F[CABLE_F_ssnow_tss], t);
+      }
+      LDS_(ssnow_smass, 3) LDS_(ssnow_ssdn, 3) LDS_(bgc_cplant, 3) LDS_(bgc_csoil, 2)
+      LDD_(drv_owb, 1) LDD_(bal_ebal_tot, 1) LDD_(bal_wbal_tot, 1) LDD_(bal_precip_tot, 1)
+      LDD_(bal_rnoff_tot, 1) LDD_(bal_evap_tot, 1) LDD_(bal_radbalsum, 1)
+      T.ssnow_gammzz[0] = ld<double>(F[CABLE_F_ssnow_gammzz], t);
+#undef LDS_
+#undef LDD_
+    }
+  };
+  if constexpr (MODE == MODE_CAPTURE) surrogate_features<R>(a, t, vrow, srow, o.feat);
+  if constexpr (MODE == MODE_HYBRID) {
+    if (a.pred) {
+#pragma unroll
+      for (int c = 0; c < mlp::NOUT; c++) S.pred[c] = a.pred[t * mlp::NOUT + c];
+    }
+  }
+  // the offline driver's running carbon sums (driver mode): the resident
+  // run reads the previous step's sums only after soil and snow
+  auto tail = [&]() {
+    if constexpr (!FULL) {
+      if (a.driver) {
+        void *const *F = a.F;
+        using DT = typename DevT<R, 'F'>::type;
+        __asm__ __volatile__("" ::: "memory");
+#define LDF_(id) T.id[0] = ld<DT>(F[CABLE_F_##id], t);
+        LDF_(sum_flux_sumpn) LDF_(sum_flux_sumrp) LDF_(sum_flux_sumrpw) LDF_(sum_flux_sumrpr)
+        LDF_(sum_flux_sumrs) LDF_(sum_flux_sumrd) LDF_(sum_flux_dsumpn) LDF_(sum_flux_dsumrp)
+        LDF_(sum_flux_dsumrd)
+#undef LDF_
+      }
+    }
+  };
+  S.run(late, tail);
+  if (a.driver) S.sumcflux();
+  o.passes = S.npass;
+  if constexpr (CONS) {
+    o.cons[0] = (double)T.bal_wbal[0];
+    o.cons[1] = (double)T.bal_ebal[0];
+    o.cons[2] = (double)T.canopy_fpn[0];
+    o.cons[3] = (double)T.canopy_fe[0];
+  }
+  if constexpr (MODE == MODE_HYBRID) o.fallback = S.vegetated && S.resid > a.resid_max;
+  if constexpr (MODE == MODE_CAPTURE) {
+    o.emit = S.vegetated;
+#pragma unroll
+    for (int c = 0; c < mlp::NOUT; c++) o.tgt[c] = S.tgt[c];
+  }
+  // the previous-step canopy state the surrogate's features read
+  if constexpr (MODE != MODE_PLAIN) {
+    if (a.hyb && !o.fallback) {
+      a.hyb[t] = (float)S.leaf_temperature;
+      a.hyb[mp + t] = (float)T.met_tvair[0];
+      a.hyb[2 * mp + t] = (float)T.met_qvair[0];
+    }
+  }
+  // ---- scatter (a tile that falls back keeps its old state) ----
+  if (o.fallback) {
+  } else if constexpr (FULL) {
+#define PARAM ROLE_PARAM
+#define FORC ROLE_FORC
+#define STATE ROLE_STATE
+#define DIAG ROLE_DIAG
+#define X(id, name, prec, n, role)                                                      \
+  if (role == ROLE_STATE || role == ROLE_DIAG || CABLE_F_##id == CABLE_F_met_tvair ||   \
+      CABLE_F_##id == CABLE_F_veg_vlai) {                                               \
+    using DT = typename DevT<R, #prec[0]>::type;                                        \
+    const int64_t off = (role == ROLE_FORC) ? a.foff : 0;                               \
+    _Pragma("unroll") for (int k = 0; k < n; k++)                                       \
+        st<DT>(F[CABLE_F_##id], (int64_t)k * mp + t + off, T.id[k]);                    \
+  }
+    CABLE_FIELDS(X)
+#undef X
+#undef PARAM
+#undef FORC
+#undef STATE
+#undef DIAG
+  } else {
+    using DT = typename DevT<R, 'F'>::type;
+#define STS_(id, n) \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) st<DT>(F[CABLE_F_##id], (int64_t)k * mp + t, T.id[k]); }
+#define STD_(id, n) \
+  { _Pragma("unroll") for (int k = 0; k < n; k++) st<double>(F[CABLE_F_##id], (int64_t)k * mp + t, T.id[k]); }
+    st<int>(F[CABLE_F_ssnow_isflag], t, T.ssnow_isflag[0]);
+    STS_(ssnow_pudsto, 1) STS_(ssnow_rtsoil, 1) STS_(ssnow_wbtot1, 1) STS_(ssnow_wbtot2, 1)
+    STS_(ssnow_wb_lake, 1) STS_(ssnow_owetfac, 1) STS_(ssnow_snage, 1) STS_(ssnow_snowd, 1)
+    STS_(ssnow_ssdnn, 1) STS_(ssnow_tss, 1) STS_(ssnow_smass, 3) STS_(ssnow_ssdn, 3)
+    STS_(ssnow_tgg, 6) STS_(ssnow_tggsn, 3) STS_(canopy_cansto, 1) STS_(bgc_cplant, 3)
+    STS_(bgc_csoil, 2)
+    STD_(ssnow_wb, 6) STD_(ssnow_wbice, 6) STD_(drv_owb, 1)
+    STD_(bal_ebal_tot, 1) STD_(bal_wbal_tot, 1) STD_(bal_precip_tot, 1) STD_(bal_rnoff_tot, 1)
+    STD_(bal_evap_tot, 1) STD_(bal_radbalsum, 1)
+    st<double>(F[CABLE_F_ssnow_gammzz], t, T.ssnow_gammzz[0]);
+    if (a.driver) {
+      STS_(sum_flux_sumpn, 1) STS_(sum_flux_sumrp, 1) STS_(sum_flux_sumrpw, 1) STS_(sum_flux_sumrpr, 1)
+      STS_(sum_flux_sumrs, 1) STS_(sum_flux_sumrd, 1) STS_(sum_flux_dsumpn, 1) STS_(sum_flux_dsumrp, 1)
+      STS_(sum_flux_dsumrd, 1)
+    }
+    // the eight headline fluxes
+    STS_(canopy_rnet, 1) STS_(canopy_fe, 1) STS_(canopy_fh, 1) STS_(canopy_fev, 1)
+    STD_(canopy_fes, 1) STS_(canopy_fgpp, 1) STS_(ssnow_runoff, 1) STS_(rad_trad, 1)
+#undef STS_
+#undef STD_
+  }
+}
+
 // CONS: also form the workgroup's sums for the driver's consistency check
 // (a separate instantiation: the check's code costs the plain step 2-3% even
 // when it is switched off at run time)
@@ -147,235 +379,19 @@ cbm_kernel(StepArgs a) {
     live = gi < (int64_t)*a.ntiles;
     t = live ? (int64_t)a.tile_index[gi] : 0;
   }
-  int passes = 0;
-  bool fallback = false, emit = false;
-  double cons[CONS ? 4 : 1] = {0.0};   // this tile's wbal, ebal, fpn, fe (CONS)
-  float feat[MODE == MODE_CAPTURE ? mlp::NIN : 1];
-  float tgt[MODE == MODE_CAPTURE ? mlp::NOUT : 1];
-  if (live) {
-    void *const *F = a.F;
-    Tile<R> T;
-    // ---- gather ----
-    if constexpr (FULL) {
-#define PARAM ROLE_PARAM
-#define FORC ROLE_FORC
-#define STATE ROLE_STATE
-#define DIAG ROLE_DIAG
-#define X(id, name, prec, n, role)                                                    \
-  {                                                                                   \
-    using DT = typename DevT<R, #prec[0]>::type;                                      \
-    const int64_t off = (role == ROLE_FORC) ? a.foff : 0;                             \
-    _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] =                           \
-        (decltype(T.id[0] + 0))ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t + off);     \
-  }
-      CABLE_FIELDS(X)
-#undef X
-#undef PARAM
-#undef FORC
-#undef STATE
-#undef DIAG
-    } else {
-      // prognostic state
-#define LDS_(id, n)                                                                   \
-  {                                                                                   \
-    using DT = typename DevT<R, 'F'>::type;                                           \
-    _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t); \
-  }
-#define LDD_(id, n)                                                                   \
-  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<double>(F[CABLE_F_##id], (int64_t)k * mp + t); }
-      T.ssnow_isflag[0] = ld<int>(F[CABLE_F_ssnow_isflag], t);
-      LDS_(ssnow_pudsto, 1) LDS_(ssnow_rtsoil, 1) LDS_(ssnow_wbtot1, 1) LDS_(ssnow_wbtot2, 1)
-      LDS_(ssnow_wb_lake, 1) LDS_(ssnow_owetfac, 1) LDS_(ssnow_snage, 1) LDS_(ssnow_snowd, 1)
-      LDS_(ssnow_ssdnn, 1) LDS_(ssnow_tss, 1)
-      LDS_(ssnow_tgg, 1) LDS_(ssnow_tggsn, 1) LDS_(canopy_cansto, 1)
-      LDD_(ssnow_wb, 1) LDD_(ssnow_wbice, 1)
-      // the canopy reads only the top soil and snow layer; the deeper layers,
-      // snow layers, carbon pools, gammzz(:,1) and the balance accumulators
-      // are loaded after the canopy (Stepper::run's `late`).  Soil liquid
-      // water as the previous step left it (wb - wbice), before this step's
-      // lake fix-up:
-#pragma unroll
-      for (int k = 0; k < MS; k++)
-        T.ssnow_wbliq[k] = ld<double>(F[CABLE_F_ssnow_wb], (int64_t)k * mp + t) -
-                           ld<double>(F[CABLE_F_ssnow_wbice], (int64_t)k * mp + t);
-      // per-tile parameters
-      T.veg_iveg[0] = ld<int>(F[CABLE_F_veg_iveg], t);
-      T.soil_isoilm[0] = ld<int>(F[CABLE_F_soil_isoilm], t);
-      LDS_(veg_vlai, 1)
-      T.soil_albsoil[0] = ld<typename DevT<R, 'F'>::type>(F[CABLE_F_soil_albsoil], t);
-      // compact forcing (offline reader's derivations)
-      {
-        using DT = typename DevT<R, 'F'>::type;
-        const int64_t i = t + a.foff;
-        R fsd = ld<DT>(F[CABLE_F_met_fsd], i);
-        T.met_fsd[0] = fsd;
-        T.met_fsd[1] = fsd;
-        R tk = ld<DT>(F[CABLE_F_met_tk], i);
-        T.met_tk[0] = tk;
-        T.met_tvair[0] = tk;
-        T.met_tvrad[0] = tk;
-        T.met_fld[0] = ld<DT>(F[CABLE_F_met_fld], i);
-        T.met_qv[0] = ld<DT>(F[CABLE_F_met_qv], i);
-        T.met_ua[0] = ld<DT>(F[CABLE_F_met_ua], i);
-        T.met_pmb[0] = ld<DT>(F[CABLE_F_met_pmb], i);
-        R pr = ld<DT>(F[CABLE_F_met_precip], i);
-        T.met_precip[0] = pr;
-        T.met_precip_sn[0] = (tk <= R(273.16)) ? pr : R(0.0);
-        T.met_coszen[0] = ld<DT>(F[CABLE_F_met_coszen], i);
-        T.met_ca[0] = (R)a.ca;
-        T.met_doy[0] = (R)a.doy;
-      }
-#undef LDS_
-#undef LDD_
-    }
-    const int iv = T.veg_iveg[0] - 1, is = T.soil_isoilm[0] - 1;
-    const double *vrow = TABLES ? lds_v + (iv < 0 ? 0 : iv) * VC_N : nullptr;
-    const double *srow = TABLES ? lds_s + (is < 0 ? 0 : is) * SC_N : nullptr;
-    Stepper<R, FULL, TABLES, MODE> S(T, a, vrow, srow);
-    auto late = [&]() {
-      if constexpr (!FULL) {
-        void *const *F = a.F;
-        using DT = typename DevT<R, 'F'>::type;
-#define LDS_(id, n) \
-  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<DT>(F[CABLE_F_##id], (int64_t)k * mp + t); }
-#define LDD_(id, n) \
-  { _Pragma("unroll") for (int k = 0; k < n; k++) T.id[k] = ld<double>(F[CABLE_F_##id], (int64_t)k * mp + t); }
-        // a fresh read: nothing has written these since the gather, and the
-        // compiler must not keep the gather's values live to reuse them
-        __asm__ __volatile__("" ::: "memory");
-#define LDL_(T_, id, k0, n) \
-  { _Pragma("unroll") for (int k = k0; k < n; k++) T.id[k] = ld<T_>(F[CABLE_F_##id], (int64_t)k * mp + t); }
-        LDL_(DT, ssnow_tgg, 1, 6) LDL_(DT, ssnow_tggsn, 1, 3) LDL_(double, ssnow_wb, 1, 6)
-        LDL_(double, ssnow_wbice, 1, 6)
-#undef LDL_
-        // values the canopy reads only before its stability iterations and
-        // soil, snow and the balances read again: the shortwave forcing, the
-        // precipitation and its snow part, and the step's starting surface
-        // temperature (otss, which the step sets from the stored tss).  FP64
-        // only: the FP32 kernel's register allocation spills more with it
-        if constexpr (sizeof(R) == 8) {
-          const int64_t i = t + a.foff;
-          const R fsd = ld<DT>(F[CABLE_F_met_fsd], i), pr = ld<DT>(F[CABLE_F_met_precip], i);
-          T.met_fsd[0] = fsd;
-          T.met_fsd[1] = fsd;
-          T.met_precip[0] = pr;
-          T.met_precip_sn[0] = (T.met_tk[0] <= R(273.16)) ? pr : R(0.0);
-          T.ssnow_otss[0] = ld<DT>(F[CABLE_F_ssnow_tss], t);
-        }
-        LDS_(ssnow_smass, 3) LDS_(ssnow_ssdn, 3) LDS_(bgc_cplant, 3) LDS_(bgc_csoil, 2)
-        LDD_(drv_owb, 1) LDD_(bal_ebal_tot, 1) LDD_(bal_wbal_tot, 1) LDD_(bal_precip_tot, 1)
-        LDD_(bal_rnoff_tot, 1) LDD_(bal_evap_tot, 1) LDD_(bal_radbalsum, 1)
-        T.ssnow_gammzz[0] = ld<double>(F[CABLE_F_ssnow_gammzz], t);
-#undef LDS_
-#undef LDD_
-      }
-    };
-    if constexpr (MODE == MODE_CAPTURE) surrogate_features<R>(a, t, vrow, srow, feat);
-    if constexpr (MODE == MODE_HYBRID) {
-      if (a.pred) {
-#pragma unroll
-        for (int c = 0; c < mlp::NOUT; c++) S.pred[c] = a.pred[t * mlp::NOUT + c];
-      }
-    }
-    // the offline driver's running carbon sums (driver mode): the resident
-    // run reads the previous step's sums only after soil and snow
-    auto tail = [&]() {
-      if constexpr (!FULL) {
-        if (a.driver) {
-          void *const *F = a.F;
-          using DT = typename DevT<R, 'F'>::type;
-          __asm__ __volatile__("" ::: "memory");
-#define LDF_(id) T.id[0] = ld<DT>(F[CABLE_F_##id], t);
-          LDF_(sum_flux_sumpn) LDF_(sum_flux_sumrp) LDF_(sum_flux_sumrpw) LDF_(sum_flux_sumrpr)
-          LDF_(sum_flux_sumrs) LDF_(sum_flux_sumrd) LDF_(sum_flux_dsumpn) LDF_(sum_flux_dsumrp)
-          LDF_(sum_flux_dsumrd)
-#undef LDF_
-        }
-      }
-    };
-    S.run(late, tail);
-    if (a.driver) S.sumcflux();
-    passes = S.npass;
-    if constexpr (CONS) {
-      cons[0] = (double)T.bal_wbal[0];
-      cons[1] = (double)T.bal_ebal[0];
-      cons[2] = (double)T.canopy_fpn[0];
-      cons[3] = (double)T.canopy_fe[0];
-    }
-    if constexpr (MODE == MODE_HYBRID) fallback = S.vegetated && S.resid > a.resid_max;
-    if constexpr (MODE == MODE_CAPTURE) {
-      emit = S.vegetated;
-#pragma unroll
-      for (int c = 0; c < mlp::NOUT; c++) tgt[c] = S.tgt[c];
-    }
-    // the previous-step canopy state the surrogate's features read
-    if constexpr (MODE != MODE_PLAIN) {
-      if (a.hyb && !fallback) {
-        a.hyb[t] = (float)S.leaf_temperature;
-        a.hyb[mp + t] = (float)T.met_tvair[0];
-        a.hyb[2 * mp + t] = (float)T.met_qvair[0];
-      }
-    }
-    // ---- scatter (a tile that falls back keeps its old state) ----
-    if (fallback) {
-    } else if constexpr (FULL) {
-#define PARAM ROLE_PARAM
-#define FORC ROLE_FORC
-#define STATE ROLE_STATE
-#define DIAG ROLE_DIAG
-#define X(id, name, prec, n, role)                                                      \
-  if (role == ROLE_STATE || role == ROLE_DIAG || CABLE_F_##id == CABLE_F_met_tvair ||   \
-      CABLE_F_##id == CABLE_F_veg_vlai) {                                               \
-    using DT = typename DevT<R, #prec[0]>::type;                                        \
-    const int64_t off = (role == ROLE_FORC) ? a.foff : 0;                               \
-    _Pragma("unroll") for (int k = 0; k < n; k++)                                       \
-        st<DT>(F[CABLE_F_##id], (int64_t)k * mp + t + off, T.id[k]);                    \
-  }
-      CABLE_FIELDS(X)
-#undef X
-#undef PARAM
-#undef FORC
-#undef STATE
-#undef DIAG
-    } else {
-      using DT = typename DevT<R, 'F'>::type;
-#define STS_(id, n) \
-  { _Pragma("unroll") for (int k = 0; k < n; k++) st<DT>(F[CABLE_F_##id], (int64_t)k * mp + t, T.id[k]); }
-#define STD_(id, n) \
-  { _Pragma("unroll") for (int k = 0; k < n; k++) st<double>(F[CABLE_F_##id], (int64_t)k * mp + t, T.id[k]); }
-      st<int>(F[CABLE_F_ssnow_isflag], t, T.ssnow_isflag[0]);
-      STS_(ssnow_pudsto, 1) STS_(ssnow_rtsoil, 1) STS_(ssnow_wbtot1, 1) STS_(ssnow_wbtot2, 1)
-      STS_(ssnow_wb_lake, 1) STS_(ssnow_owetfac, 1) STS_(ssnow_snage, 1) STS_(ssnow_snowd, 1)
-      STS_(ssnow_ssdnn, 1) STS_(ssnow_tss, 1) STS_(ssnow_smass, 3) STS_(ssnow_ssdn, 3)
-      STS_(ssnow_tgg, 6) STS_(ssnow_tggsn, 3) STS_(canopy_cansto, 1) STS_(bgc_cplant, 3)
-      STS_(bgc_csoil, 2)
-      STD_(ssnow_wb, 6) STD_(ssnow_wbice, 6) STD_(drv_owb, 1)
-      STD_(bal_ebal_tot, 1) STD_(bal_wbal_tot, 1) STD_(bal_precip_tot, 1) STD_(bal_rnoff_tot, 1)
-      STD_(bal_evap_tot, 1) STD_(bal_radbalsum, 1)
-      st<double>(F[CABLE_F_ssnow_gammzz], t, T.ssnow_gammzz[0]);
-      if (a.driver) {
-        STS_(sum_flux_sumpn, 1) STS_(sum_flux_sumrp, 1) STS_(sum_flux_sumrpw, 1) STS_(sum_flux_sumrpr, 1)
-        STS_(sum_flux_sumrs, 1) STS_(sum_flux_sumrd, 1) STS_(sum_flux_dsumpn, 1) STS_(sum_flux_dsumrp, 1)
-        STS_(sum_flux_dsumrd, 1)
-      }
-      // the eight headline fluxes
-      STS_(canopy_rnet, 1) STS_(canopy_fe, 1) STS_(canopy_fh, 1) STS_(canopy_fev, 1)
-      STD_(canopy_fes, 1) STS_(canopy_fgpp, 1) STS_(ssnow_runoff, 1) STS_(rad_trad, 1)
-#undef STS_
-#undef STD_
-    }
-  }
+  TileOut<MODE, CONS> o;
+  if (live) step_tile<R, FULL, TABLES, MODE, CONS>(a, t, lds_v, lds_s, o);
   if constexpr (CONS) {
     // workgroup sums in a fixed order: a butterfly within each wave, then
     // the waves in order
     __shared__ double wsum[4][4];
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      for (int o = 32; o > 0; o >>= 1) cons[q] += __shfl_xor(cons[q], o, 64);
+      for (int k = 32; k > 0; k >>= 1) o.cons[q] += __shfl_xor(o.cons[q], k, 64);
     const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     if ((threadIdx.x & 63) == 0)
 #pragma unroll
-      for (int q = 0; q < 4; q++) wsum[w][q] = cons[q];
+      for (int q = 0; q < 4; q++) wsum[w][q] = o.cons[q];
     __syncthreads();
     if (threadIdx.x < 4) {
       double s = 0.0;
@@ -385,20 +401,21 @@ cbm_kernel(StepArgs a) {
   }
   if (a.npass) {
     // wave-level sum, one atomic per wave
-    for (int o = 32; o > 0; o >>= 1) passes += __shfl_xor(passes, o, 64);
+    int passes = o.passes;
+    for (int k = 32; k > 0; k >>= 1) passes += __shfl_xor(passes, k, 64);
     if ((threadIdx.x & 63) == 0 && passes) atomicAdd(a.npass, (unsigned long long)passes);
   }
   if constexpr (MODE == MODE_HYBRID) {
-    const unsigned slot = wave_append32(fallback, a.qlen);
-    if (fallback) a.queue[slot] = (int)t;   // the queue holds mp entries
+    const unsigned slot = wave_append32(o.fallback, a.qlen);
+    if (o.fallback) a.queue[slot] = (int)t;   // the queue holds mp entries
   }
   if constexpr (MODE == MODE_CAPTURE) {
-    const unsigned long long slot = wave_append(emit, a.scount);
-    if (emit && slot < a.scap) {
+    const unsigned long long slot = wave_append(o.emit, a.scount);
+    if (o.emit && slot < a.scap) {
 #pragma unroll
-      for (int c = 0; c < mlp::NIN; c++) a.sx[slot * mlp::NIN + c] = feat[c];
+      for (int c = 0; c < mlp::NIN; c++) a.sx[slot * mlp::NIN + c] = o.feat[c];
 #pragma unroll
-      for (int c = 0; c < mlp::NOUT; c++) a.sy[slot * mlp::NOUT + c] = tgt[c];
+      for (int c = 0; c < mlp::NOUT; c++) a.sy[slot * mlp::NOUT + c] = o.tgt[c];
     }
   }
 }
