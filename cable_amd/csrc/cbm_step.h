@@ -425,6 +425,22 @@ struct Stepper {
     return MAX(K(0.0), num / den);
   }
 
+  // The larger root of coef2 z^2 + coef1 z + coef0 = 0 as the reference
+  // takes it, (-coef1 + sqrt(max(0, delcx))) / (2 coef2) with delcx =
+  // coef1^2 - 4 coef0 coef2 (cbl_photosynthesis.F90:107-111, 157-161,
+  // 202-206).  The reference holds these coefficients in REAL(r_2) (FP64,
+  // :36-38).  In FP32 (the mixed path) the sum -coef1 + sqrt(delcx) cancels
+  // when coef1 > 0 and loses the root's digits, so there the equal
+  // 2 coef0 / (-coef1 - sqrt(delcx)) is taken, which keeps them: the
+  // canopy's GPP then stays as close to FP64 as the reference's own shipped
+  // build does.  FP64 keeps the reference's form.
+  CBM_HD static R larger_root(R coef2z, R coef1z, R coef0z, R delcxz) {
+    if constexpr (sizeof(R) == 4) {
+      if (coef1z > K(0.0)) return (K(2.0) * coef0z) / (-coef1z - sqrt(MAX(K(0.0), delcxz)));
+    }
+    return (-coef1z + sqrt(MAX(K(0.0), delcxz))) / (K(2.0) * coef2z);
+  }
+
   // src/science/canopy/cbl_photosynthesis.F90:10-226, one leaf
   CBM_HD static R photosynthesis_leaf(R csxz, R cx1z, R cx2z, R gswminz, R rdxz, R vcmxt3z,
                                           R vcmxt4z, R vx3z, R vx4z, R gs_coeffz, R fwsoilz) {
@@ -448,7 +464,7 @@ struct Stepper {
     }
     if (fabs(coef2z) >= K(1.e-9)) {
       delcxz = p2(coef1z) - K(4.0) * coef0z * coef2z;
-      ciz = (-coef1z + sqrt(MAX(K(0.0), delcxz))) / (K(2.0) * coef2z);
+      ciz = larger_root(coef2z, coef1z, coef0z, delcxz);
       ciz = MAX(K(0.0), ciz);
       anrubiscoz = vcmxt3z * (ciz - cx2z / K(2.0)) / (ciz + cx1z) + vcmxt4z - rdxz;
     }
@@ -466,7 +482,7 @@ struct Stepper {
     }
     if (fabs(coef2z) >= K(1.e-9)) {
       delcxz = p2(coef1z) - K(4.0) * coef0z * coef2z;
-      ciz = (-coef1z + sqrt(MAX(K(0.0), delcxz))) / (K(2.0) * coef2z);
+      ciz = larger_root(coef2z, coef1z, coef0z, delcxz);
       ciz = MAX(K(0.0), ciz);
       anrubpz = vx3z * (ciz - cx2z / K(2.0)) / (ciz + cx2z) + vx4z - rdxz;
     }
@@ -483,7 +499,7 @@ struct Stepper {
     }
     if (fabs(coef2z) >= K(1.e-9)) {
       delcxz = p2(coef1z) - K(4.0) * coef0z * coef2z;
-      ciz = (-coef1z + sqrt(MAX(K(0.0), delcxz))) / (K(2.0) * coef2z);
+      ciz = larger_root(coef2z, coef1z, coef0z, delcxz);
       ansinkz = ciz;
     }
     return MIN(MIN(anrubiscoz, anrubpz), ansinkz);
@@ -949,8 +965,9 @@ struct Stepper {
     t.canopy_fns_cor[0] = K(0.0);
     t.canopy_ga_cor[0] = K(0.0);
     radiation(sunlit_veg);
-    // stability parameter of each NITER pass, kept in registers (zetar(:,1..4))
-    R zeta1 = ZETPOS + K(1), zeta2 = t.canopy_zetar[2], zeta3 = t.canopy_zetar[3];
+    // stability parameter of each NITER pass, kept in registers (zetar(:,1..4));
+    // passes 1-3 set zeta1-zeta3 before passes 2-4 read them
+    R zeta1 = ZETPOS + K(1), zeta2 = ZETA0, zeta3 = ZETA0;
     t.canopy_zetash[0] = ZETA0;
     t.canopy_zetash[1] = ZETPOS + K(1);
     c.sum_rad_rniso = t.rad_rniso[0] + t.rad_rniso[1];
