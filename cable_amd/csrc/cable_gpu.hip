@@ -93,6 +93,10 @@ struct cable_gpu_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
+  // per-launch events of the hybrid run (a pair per step and kernel), and
+  // the device time of its step and inference kernels
+  std::vector<hipEvent_t> kev;
+  double last_step_ms = 0.0, last_infer_ms = 0.0;
   int device = 0;
   // device position j holds the caller's tile order[j] (empty: identity)
   std::vector<int64_t> order;
@@ -105,9 +109,7 @@ struct cable_gpu_ctx {
   int64_t scap = 0;
   float *sparams = nullptr, *snorm = nullptr, *pred = nullptr;  // network, [mp][8]
   uint16_t *swt = nullptr;          // transposed BF16 weights, per layer [out][in]
-  int *queue = nullptr;             // fallback tiles of a step
-  unsigned *qlen = nullptr;
-  unsigned long long *qtotal = nullptr;
+  unsigned long long *nfallback = nullptr;  // tiles that fell back, summed over a run
   int64_t last_fallbacks = 0, last_tile_steps = 0;
   // ---- the driver's consistency check ----
   bool cons_on = false;
@@ -259,12 +261,13 @@ int cable_gpu_destroy(cable_gpu_ctx *c) {
   if (c->d_vtab) hipFree(c->d_vtab);
   if (c->d_stab) hipFree(c->d_stab);
   if (c->d_npass) hipFree(c->d_npass);
-  void *sur[] = {c->cons_part, c->cons_acc, c->hyb, c->sx, c->sy, c->scount, c->sparams, c->snorm, c->pred, c->swt, c->queue, c->qlen,
-                 c->qtotal};
+  void *sur[] = {c->cons_part, c->cons_acc, c->hyb, c->sx, c->sy, c->scount, c->sparams, c->snorm, c->pred, c->swt,
+                 c->nfallback};
   for (void *p : sur)
     if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (hipEvent_t ev : c->kev) hipEventDestroy(ev);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return CABLE_OK;
@@ -1064,6 +1067,8 @@ int cable_gpu_run(cable_gpu_ctx *c, int k0, int nsteps, double dels) {
   float ms = 0.f;
   HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  c->last_step_ms = ms;
+  c->last_infer_ms = 0.0;
   unsigned long long np = 0;
   HIPCHECK(hipMemcpy(&np, c->d_npass, sizeof np, hipMemcpyDeviceToHost));
   c->last_passes = (int64_t)np;
@@ -1181,10 +1186,13 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
     HIPCHECK(hipMalloc(&c->pred, sizeof(float) * cbm_mlp::NOUT * (size_t)c->mp));
     HIPCHECK(hipMemset(c->pred, 0, sizeof(float) * cbm_mlp::NOUT * (size_t)c->mp));
   }
-  if (!c->queue) HIPCHECK(hipMalloc(&c->queue, sizeof(int) * (size_t)c->mp));
-  if (!c->qlen) HIPCHECK(hipMalloc(&c->qlen, sizeof(unsigned)));
-  if (!c->qtotal) HIPCHECK(hipMalloc(&c->qtotal, sizeof(unsigned long long)));
-  HIPCHECK(hipMemsetAsync(c->qtotal, 0, sizeof(unsigned long long), c->stream));
+  if (!c->nfallback) HIPCHECK(hipMalloc(&c->nfallback, sizeof(unsigned long long)));
+  while (c->kev.size() < 4 * (size_t)nsteps) {
+    hipEvent_t ev;
+    HIPCHECK(hipEventCreate(&ev));
+    c->kev.push_back(ev);
+  }
+  HIPCHECK(hipMemsetAsync(c->nfallback, 0, sizeof(unsigned long long), c->stream));
   HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
   for (int k = k0; k < k0 + nsteps; k++) {
@@ -1195,20 +1203,17 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
     a.pred = use_surrogate ? c->pred : nullptr;
     a.max_passes = max_passes;
     a.resid_max = (float)resid_max;
-    a.queue = c->queue;
-    a.qlen = c->qlen;
-    HIPCHECK(hipMemsetAsync(c->qlen, 0, sizeof(unsigned), c->stream));
+    a.nfallback = c->nfallback;
     hipError_t e = hipSuccess;
+    hipEvent_t *ev = &c->kev[4 * (size_t)(k - k0)];
+    HIPCHECK(hipEventRecord(ev[0], c->stream));
     if (use_surrogate) e = launch_mlp_infer(a, c->swt, c->sparams, c->pred, nullptr, c->mp, c->stream);
+    HIPCHECK(hipEventRecord(ev[1], c->stream));
+    // the capped step; a tile the residual test rejects falls back to the
+    // reference's iteration within the same launch
+    HIPCHECK(hipEventRecord(ev[2], c->stream));
     if (e == hipSuccess) e = launch_mixed_mode(MODE_HYBRID, a, c->mp, c->stream);
-    // the tiles the capped iteration did not settle: the plain method, in
-    // place, from the state the hybrid step left untouched for them
-    StepArgs f = a;
-    f.tile_index = c->queue;
-    f.ntiles = c->qlen;
-    f.pred = nullptr;
-    if (e == hipSuccess) e = launch_mixed_mode(MODE_FALLBACK, f, c->mp, c->stream);
-    if (e == hipSuccess) e = launch_count(c->qlen, c->qtotal, c->stream);
+    HIPCHECK(hipEventRecord(ev[3], c->stream));
     if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run_hybrid launch: %s", hipGetErrorString(e));
     c->first_call = false;
     c->hyb_valid = true;
@@ -1220,7 +1225,15 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
   c->last_ms = ms;
   unsigned long long np = 0, fb = 0;
   HIPCHECK(hipMemcpy(&np, c->d_npass, sizeof np, hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(&fb, c->qtotal, sizeof fb, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(&fb, c->nfallback, sizeof fb, hipMemcpyDeviceToHost));
+  c->last_infer_ms = c->last_step_ms = 0.0;
+  for (int k = 0; k < nsteps; k++) {
+    float a = 0.f, b = 0.f;
+    HIPCHECK(hipEventElapsedTime(&a, c->kev[4 * (size_t)k], c->kev[4 * (size_t)k + 1]));
+    HIPCHECK(hipEventElapsedTime(&b, c->kev[4 * (size_t)k + 2], c->kev[4 * (size_t)k + 3]));
+    c->last_infer_ms += a;
+    c->last_step_ms += b;
+  }
   c->last_passes = (int64_t)np;
   c->last_fallbacks = (int64_t)fb;
   c->last_tile_steps = (int64_t)nsteps * c->mp;
@@ -1274,6 +1287,13 @@ int cable_gpu_sync(cable_gpu_ctx *c) {
 int cable_gpu_last_run_ms(cable_gpu_ctx *c, double *ms) {
   if (!c || !ms) return fail(CABLE_E_ARG, "cable_gpu_last_run_ms: null argument");
   *ms = c->last_ms;
+  return CABLE_OK;
+}
+
+int cable_gpu_last_kernel_ms(cable_gpu_ctx *c, double *step_ms, double *infer_ms) {
+  if (!c || !step_ms || !infer_ms) return fail(CABLE_E_ARG, "cable_gpu_last_kernel_ms: null argument");
+  *step_ms = c->last_step_ms;
+  *infer_ms = c->last_infer_ms;
   return CABLE_OK;
 }
 

@@ -112,20 +112,13 @@ __device__ inline unsigned long long wave_append(bool take, unsigned long long *
   base = __shfl(base, 0, 64);
   return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
-__device__ inline unsigned wave_append32(bool take, unsigned *count) {
-  const unsigned long long mask = __ballot(take);
-  const int lane = threadIdx.x & 63;
-  unsigned base = 0;
-  if (lane == 0 && mask) base = atomicAdd(count, (unsigned)__popcll(mask));
-  base = __shfl(base, 0, 64);
-  return base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-}
 
 // what a tile's step reports to the kernel around it
 template <int MODE, bool CONS>
 struct TileOut {
   int passes = 0;                      // dryLeaf passes (work counter)
-  bool fallback = false;               // MODE_HYBRID: the tile must be stepped again
+  bool fallback = false;               // MODE_HYBRID: the tile fell back to the
+                                       // reference's leaf iteration
   bool emit = false;                   // MODE_CAPTURE: the tile yields a sample
   double cons[CONS ? 4 : 1] = {0.0};   // CONS: the tile's wbal, ebal, fpn, fe
   float feat[MODE == MODE_CAPTURE ? mlp::NIN : 1];
@@ -290,7 +283,7 @@ CBM_HD void step_tile(const StepArgs &a, int64_t t, const double *vtab, const do
     o.cons[2] = (double)T.canopy_fpn[0];
     o.cons[3] = (double)T.canopy_fe[0];
   }
-  if constexpr (MODE == MODE_HYBRID) o.fallback = S.vegetated && S.resid > a.resid_max;
+  if constexpr (MODE == MODE_HYBRID) o.fallback = S.fell_back;
   if constexpr (MODE == MODE_CAPTURE) {
     o.emit = S.vegetated;
 #pragma unroll
@@ -298,15 +291,14 @@ CBM_HD void step_tile(const StepArgs &a, int64_t t, const double *vtab, const do
   }
   // the previous-step canopy state the surrogate's features read
   if constexpr (MODE != MODE_PLAIN) {
-    if (a.hyb && !o.fallback) {
+    if (a.hyb) {
       a.hyb[t] = (float)S.leaf_temperature;
       a.hyb[mp + t] = (float)T.met_tvair[0];
       a.hyb[2 * mp + t] = (float)T.met_qvair[0];
     }
   }
-  // ---- scatter (a tile that falls back keeps its old state) ----
-  if (o.fallback) {
-  } else if constexpr (FULL) {
+  // ---- scatter ----
+  if constexpr (FULL) {
 #define PARAM ROLE_PARAM
 #define FORC ROLE_FORC
 #define STATE ROLE_STATE
@@ -363,22 +355,13 @@ cbm_kernel(StepArgs a) {
   __shared__ double lds_v[TABLES ? NVEGCLASS * VC_N : 1];
   __shared__ double lds_s[TABLES ? NSOILCLASS * SC_N : 1];
   const int64_t mp = a.mp;
-  // a fallback launch covers mp threads but steps only *a.ntiles tiles:
-  // workgroups past the end leave (all their threads, before any barrier)
-  if (a.tile_index && (int64_t)blockIdx.x * blockDim.x >= (int64_t)*a.ntiles) return;
   if constexpr (TABLES) {
     for (int i = threadIdx.x; i < NVEGCLASS * VC_N; i += blockDim.x) lds_v[i] = a.vtab[i];
     for (int i = threadIdx.x; i < NSOILCLASS * SC_N; i += blockDim.x) lds_s[i] = a.stab[i];
     __syncthreads();
   }
-  // thread -> tile: direct, or through a.tile_index (the hybrid's fallback)
-  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t t = gi;
-  bool live = gi < mp;
-  if (a.tile_index) {
-    live = gi < (int64_t)*a.ntiles;
-    t = live ? (int64_t)a.tile_index[gi] : 0;
-  }
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < mp;
   TileOut<MODE, CONS> o;
   if (live) step_tile<R, FULL, TABLES, MODE, CONS>(a, t, lds_v, lds_s, o);
   if constexpr (CONS) {
@@ -406,8 +389,9 @@ cbm_kernel(StepArgs a) {
     if ((threadIdx.x & 63) == 0 && passes) atomicAdd(a.npass, (unsigned long long)passes);
   }
   if constexpr (MODE == MODE_HYBRID) {
-    const unsigned slot = wave_append32(o.fallback, a.qlen);
-    if (o.fallback) a.queue[slot] = (int)t;   // the queue holds mp entries
+    // tiles that fell back: a wave-level count, one atomic per wave
+    const unsigned long long fb = __ballot(o.fallback);
+    if ((threadIdx.x & 63) == 0 && fb) atomicAdd(a.nfallback, (unsigned long long)__popcll(fb));
   }
   if constexpr (MODE == MODE_CAPTURE) {
     const unsigned long long slot = wave_append(o.emit, a.scount);
@@ -436,6 +420,7 @@ cbm_kernel(StepArgs a) {
 namespace infer {
 using namespace mlp;
 constexpr int WAVES = 8, TILES = 32 * WAVES;
+constexpr int VCHUNK = 64;                       // blocks voted on at once
 constexpr int K1 = 32;                           // layer-1 K padded to a multiple of 16
 constexpr int P1 = K1 + 8, PH = HID + 8;         // weight row pitches (BF16)
 constexpr int AP = HID + 8;                      // activation row pitch
@@ -465,137 +450,153 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
   __shared__ float bl[BLDS];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31;
   const int kh = 8 * (lane >> 5), kq = 4 * (lane >> 5);
-  // stage the weights once per workgroup (the grid is one workgroup per CU,
-  // each stepping every gridDim.x-th block of 256 tiles): 8-byte units of
-  // four consecutive inputs, loads issued together before the stores
-  {
-    constexpr int U = GTOT / 4, PER = (U + 64 * WAVES - 1) / (64 * WAVES);
-    uint2 v[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      const int u = threadIdx.x + q * 64 * WAVES;
-      if (u < U) v[q] = *(const uint2 *)&wt[4 * u];
-    }
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      const int u = threadIdx.x + q * 64 * WAVES, e = 4 * u;
-      if (u < U) {
-        int dst;
-        if (e < G2) dst = OFF1 + (e / NIN) * P1 + e % NIN;
-        else if (e < G3) dst = OFF2 + ((e - G2) / HID) * PH + (e - G2) % HID;
-        else if (e < G4) dst = OFF3 + ((e - G3) / HID) * PH + (e - G3) % HID;
-        else dst = OFF4 + ((e - G4) / HID) * PH + (e - G4) % HID;
-        *(uint2 *)&wl[dst] = v[q];
-      }
-    }
-  }
-  for (int u = threadIdx.x; u < HID * 2; u += blockDim.x)
-    *(uint2 *)&wl[OFF1 + (u >> 1) * P1 + NIN + 4 * (u & 1)] = make_uint2(0u, 0u);
-  for (int u = threadIdx.x; u < BLDS; u += blockDim.x)
-    bl[u] = params[u < 3 * HID ? LAYER_B[u / HID] + u % HID : LAYER_B[3] + u - 3 * HID];
   __bf16 *A = &act[wave * 32 * AP];
+  bool staged = false;
   const int64_t nblk = (a.mp + TILES - 1) / TILES;
-  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-    const int64_t t0 = blk * TILES;
-    // a block with no leaf-bearing tile in daylight has nothing to predict:
-    // the warm start applies to sunlit vegetated tiles only, and the test is
-    // the mixed step's own (both shortwave bands, fsd(:,2) = fsd(:,1), above
-    // RAD_THRESH in FP32).  The vote is also the barrier between the previous
-    // block's layers and this block's feature rows.
-    {
-      const int64_t t = t0 + threadIdx.x;
-      bool leafy = false;
-      if (threadIdx.x < TILES && t < a.mp && ld<int>(a.F[CABLE_F_veg_iveg], t) < 14) {
+  // A block with no leaf-bearing tile in daylight has nothing to predict:
+  // the warm start applies to sunlit vegetated tiles only, and the test is
+  // the mixed step's own (both shortwave bands, fsd(:,2) = fsd(:,1), above
+  // RAD_THRESH in FP32).  The workgroup takes the votes of VCHUNK of its
+  // blocks at once (loads issued together), then steps the blocks with work.
+  __shared__ int vote[VCHUNK];
+  const int64_t G = gridDim.x;
+  for (int64_t c0 = blockIdx.x; c0 < nblk; c0 += G * VCHUNK) {
+    __syncthreads();  // the previous chunk's blocks are done with vote[]
+    if (threadIdx.x < VCHUNK) vote[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < VCHUNK * TILES; i += blockDim.x) {
+      const int64_t b = c0 + (int64_t)(i / TILES) * G, t = b * TILES + i % TILES;
+      if (b < nblk && t < a.mp && ld<int>(a.F[CABLE_F_veg_iveg], t) < 14) {
         const float fsd = ld<float>(a.F[CABLE_F_met_fsd], t + a.foff);
-        leafy = fsd + fsd > 0.001f;
+        if (fsd + fsd > 0.001f) vote[i / TILES] = 1;
       }
-      if (!__syncthreads_or(leafy) && !xraw) continue;
-    }
-    // features of tile threadIdx.x, normalised, as BF16 row threadIdx.x
-    if (threadIdx.x < TILES) {
-      const int64_t t = t0 + threadIdx.x;
-      __bf16 *row = &act[threadIdx.x * AP];
-      float x[NIN];
-      if (t < a.mp) {
-        const int iv = ld<int>(a.F[CABLE_F_veg_iveg], t) - 1, is = ld<int>(a.F[CABLE_F_soil_isoilm], t) - 1;
-        surrogate_features<float>(a, t, a.vtab + (iv < 0 ? 0 : iv) * VC_N, a.stab + (is < 0 ? 0 : is) * SC_N, x);
-        if (xraw) {
-#pragma unroll
-          for (int c = 0; c < NIN; c++) xraw[t * NIN + c] = x[c];
-        }
-#pragma unroll
-        for (int c = 0; c < NIN; c++) x[c] = (x[c] - a.norm[c]) / a.norm[NIN + c];
-      } else {
-#pragma unroll
-        for (int c = 0; c < NIN; c++) x[c] = 0.0f;
-      }
-#pragma unroll
-      for (int c = 0; c < NIN; c += 4) {
-        bf4 v = {(__bf16)x[c], (__bf16)x[c + 1], (__bf16)x[c + 2], (__bf16)x[c + 3]};
-        *(bf4 *)&row[c] = v;
-      }
-      *(uint2 *)&row[NIN] = make_uint2(0u, 0u);
-      *(uint2 *)&row[NIN + 4] = make_uint2(0u, 0u);
     }
     __syncthreads();
-    // from here on a wave touches only its own 32 activation rows: its LDS
-    // operations complete in order, so the layers need no workgroup barrier
+    for (int qb = 0; qb < VCHUNK; qb++) {
+      const int64_t blk = c0 + (int64_t)qb * G;
+      if (blk >= nblk) break;
+      if (!vote[qb] && !xraw) continue;
+      const int64_t t0 = blk * TILES;
+      // the previous block's layers are done with the activation rows
+      __syncthreads();
+      // the weights are staged in LDS once per workgroup (the grid is one
+      // workgroup per CU, each stepping every gridDim.x-th block of 256
+      // tiles), at its first block with work, so a night step reads none; the
+      // barrier after the feature rows below also publishes them
+      if (!staged) {
+        // 8-byte units of four consecutive inputs, loads issued together
+        // before the stores
+        constexpr int U = GTOT / 4, PER = (U + 64 * WAVES - 1) / (64 * WAVES);
+        uint2 v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+          const int u = threadIdx.x + q * 64 * WAVES;
+          if (u < U) v[q] = *(const uint2 *)&wt[4 * u];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+          const int u = threadIdx.x + q * 64 * WAVES, e = 4 * u;
+          if (u < U) {
+            int dst;
+            if (e < G2) dst = OFF1 + (e / NIN) * P1 + e % NIN;
+            else if (e < G3) dst = OFF2 + ((e - G2) / HID) * PH + (e - G2) % HID;
+            else if (e < G4) dst = OFF3 + ((e - G3) / HID) * PH + (e - G3) % HID;
+            else dst = OFF4 + ((e - G4) / HID) * PH + (e - G4) % HID;
+            *(uint2 *)&wl[dst] = v[q];
+          }
+        }
+        for (int u = threadIdx.x; u < HID * 2; u += blockDim.x)
+          *(uint2 *)&wl[OFF1 + (u >> 1) * P1 + NIN + 4 * (u & 1)] = make_uint2(0u, 0u);
+        for (int u = threadIdx.x; u < BLDS; u += blockDim.x)
+          bl[u] = params[u < 3 * HID ? LAYER_B[u / HID] + u % HID : LAYER_B[3] + u - 3 * HID];
+        staged = true;
+      }
+      // features of tile threadIdx.x, normalised, as BF16 row threadIdx.x
+      if (threadIdx.x < TILES) {
+        const int64_t t = t0 + threadIdx.x;
+        __bf16 *row = &act[threadIdx.x * AP];
+        float x[NIN];
+        if (t < a.mp) {
+          const int iv = ld<int>(a.F[CABLE_F_veg_iveg], t) - 1, is = ld<int>(a.F[CABLE_F_soil_isoilm], t) - 1;
+          surrogate_features<float>(a, t, a.vtab + (iv < 0 ? 0 : iv) * VC_N, a.stab + (is < 0 ? 0 : is) * SC_N, x);
+          if (xraw) {
+#pragma unroll
+            for (int c = 0; c < NIN; c++) xraw[t * NIN + c] = x[c];
+          }
+#pragma unroll
+          for (int c = 0; c < NIN; c++) x[c] = (x[c] - a.norm[c]) / a.norm[NIN + c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < NIN; c++) x[c] = 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < NIN; c += 4) {
+          bf4 v = {(__bf16)x[c], (__bf16)x[c + 1], (__bf16)x[c + 2], (__bf16)x[c + 3]};
+          *(bf4 *)&row[c] = v;
+        }
+        *(uint2 *)&row[NIN] = make_uint2(0u, 0u);
+        *(uint2 *)&row[NIN + 4] = make_uint2(0u, 0u);
+      }
+      __syncthreads();
+      // from here on a wave touches only its own 32 activation rows: its LDS
+      // operations complete in order, so the layers need no workgroup barrier
 #pragma unroll 1
-    for (int l = 0; l < 3; l++) {
-      const int K = l == 0 ? K1 : HID;
-      const __bf16 *W = &wl[l == 0 ? OFF1 : (l == 1 ? OFF2 : OFF3)];
-      const int pw = l == 0 ? P1 : PH;
-      v16f acc[4];
+      for (int l = 0; l < 3; l++) {
+        const int K = l == 0 ? K1 : HID;
+        const __bf16 *W = &wl[l == 0 ? OFF1 : (l == 1 ? OFF2 : OFF3)];
+        const int pw = l == 0 ? P1 : PH;
+        v16f acc[4];
 #pragma unroll
-      for (int nb = 0; nb < 4; nb++)
+        for (int nb = 0; nb < 4; nb++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[nb][r] = 0.0f;
-      for (int k0 = 0; k0 < K; k0 += 16) {
-        const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
+          for (int r = 0; r < 16; r++) acc[nb][r] = 0.0f;
+        for (int k0 = 0; k0 < K; k0 += 16) {
+          const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
 #pragma unroll
-        for (int nb = 0; nb < 4; nb++) {
-          const bf8 wv = *(const bf8 *)&W[(nb * 32 + col) * pw + k0 + kh];
-          acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xv, acc[nb], 0, 0, 0);
+          for (int nb = 0; nb < 4; nb++) {
+            const bf8 wv = *(const bf8 *)&W[(nb * 32 + col) * pw + k0 + kh];
+            acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xv, acc[nb], 0, 0, 0);
+          }
         }
+        __builtin_amdgcn_wave_barrier();
+        // lane: tile `col`; register group g of block nb: outputs nb*32 + 8g + kq + 0..3
+        const float *b = &bl[l * HID];
+#pragma unroll
+        for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+          for (int g = 0; g < 4; g++) {
+            const int j = nb * 32 + 8 * g + kq;
+            bf4 v;
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[q] = (__bf16)silu_fast(acc[nb][4 * g + q] + b[j + q]);
+            *(bf4 *)&A[col * AP + j] = v;
+          }
+        __builtin_amdgcn_wave_barrier();
       }
-      __builtin_amdgcn_wave_barrier();
-      // lane: tile `col`; register group g of block nb: outputs nb*32 + 8g + kq + 0..3
-      const float *b = &bl[l * HID];
+      // output layer: 8 outputs, rows 8..31 of the weight operand are zero
+      {
+        v16f acc;
 #pragma unroll
-      for (int nb = 0; nb < 4; nb++)
+        for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+        for (int k0 = 0; k0 < HID; k0 += 16) {
+          const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
+          bf8 wv;
+          if (col < NOUT) wv = *(const bf8 *)&wl[OFF4 + col * PH + k0 + kh];
+          else
 #pragma unroll
-        for (int g = 0; g < 4; g++) {
-          const int j = nb * 32 + 8 * g + kq;
-          bf4 v;
-#pragma unroll
-          for (int q = 0; q < 4; q++) v[q] = (__bf16)silu_fast(acc[nb][4 * g + q] + b[j + q]);
-          *(bf4 *)&A[col * AP + j] = v;
+            for (int e = 0; e < 8; e++) wv[e] = (__bf16)0.0f;
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xv, acc, 0, 0, 0);
         }
-      __builtin_amdgcn_wave_barrier();
-    }
-    // output layer: 8 outputs, rows 8..31 of the weight operand are zero
-    {
-      v16f acc;
-#pragma unroll
-      for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-      for (int k0 = 0; k0 < HID; k0 += 16) {
-        const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
-        bf8 wv;
-        if (col < NOUT) wv = *(const bf8 *)&wl[OFF4 + col * PH + k0 + kh];
-        else
-#pragma unroll
-          for (int e = 0; e < 8; e++) wv[e] = (__bf16)0.0f;
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xv, acc, 0, 0, 0);
-      }
-      // lane: tile `col`, outputs kq + 0..3 in registers 0..3
-      const int64_t t = t0 + wave * 32 + col;
-      if (t < a.mp) {
-        float4 o;
-        o.x = (acc[0] + bl[3 * HID + kq + 0]) * a.norm[2 * NIN + NOUT + kq + 0] + a.norm[2 * NIN + kq + 0];
-        o.y = (acc[1] + bl[3 * HID + kq + 1]) * a.norm[2 * NIN + NOUT + kq + 1] + a.norm[2 * NIN + kq + 1];
-        o.z = (acc[2] + bl[3 * HID + kq + 2]) * a.norm[2 * NIN + NOUT + kq + 2] + a.norm[2 * NIN + kq + 2];
-        o.w = (acc[3] + bl[3 * HID + kq + 3]) * a.norm[2 * NIN + NOUT + kq + 3] + a.norm[2 * NIN + kq + 3];
-        *(float4 *)&pred[t * NOUT + kq] = o;
+        // lane: tile `col`, outputs kq + 0..3 in registers 0..3
+        const int64_t t = t0 + wave * 32 + col;
+        if (t < a.mp) {
+          float4 o;
+          o.x = (acc[0] + bl[3 * HID + kq + 0]) * a.norm[2 * NIN + NOUT + kq + 0] + a.norm[2 * NIN + kq + 0];
+          o.y = (acc[1] + bl[3 * HID + kq + 1]) * a.norm[2 * NIN + NOUT + kq + 1] + a.norm[2 * NIN + kq + 1];
+          o.z = (acc[2] + bl[3 * HID + kq + 2]) * a.norm[2 * NIN + NOUT + kq + 2] + a.norm[2 * NIN + kq + 2];
+          o.w = (acc[3] + bl[3 * HID + kq + 3]) * a.norm[2 * NIN + NOUT + kq + 3] + a.norm[2 * NIN + kq + 3];
+          *(float4 *)&pred[t * NOUT + kq] = o;
+        }
       }
     }
   }
@@ -622,6 +623,5 @@ hipError_t launch_mixed(bool full, const StepArgs &a, int64_t mp, hipStream_t st
 hipError_t launch_mixed_mode(int mode, const StepArgs &a, int64_t mp, hipStream_t stream);
 hipError_t launch_mlp_infer(const StepArgs &a, const uint16_t *wt, const float *params, float *pred,
                             float *xraw, int64_t mp, hipStream_t stream);
-hipError_t launch_count(const unsigned *qlen, unsigned long long *total, hipStream_t stream);
 
 }  // namespace cbm

@@ -17,17 +17,8 @@ hipError_t launch_mixed_mode(int mode, const StepArgs &a, int64_t mp, hipStream_
   switch (mode) {
     case MODE_CAPTURE: return launch_cbm<float, false, true, MODE_CAPTURE>(a, mp, stream);
     case MODE_HYBRID: return launch_cbm<float, false, true, MODE_HYBRID>(a, mp, stream);
-    case MODE_FALLBACK: return launch_cbm<float, false, true, MODE_FALLBACK>(a, mp, stream);
     default: return launch_cbm<float, false, true, MODE_PLAIN>(a, mp, stream);
   }
-}
-
-// total of the per-step fallback counts (one thread)
-__global__ void count_kernel(const unsigned *qlen, unsigned long long *total) { *total += *qlen; }
-
-hipError_t launch_count(const unsigned *qlen, unsigned long long *total, hipStream_t stream) {
-  hipLaunchKernelGGL(count_kernel, dim3(1), dim3(1), 0, stream, qlen, total);
-  return hipGetLastError();
 }
 
 hipError_t launch_mlp_infer(const StepArgs &a, const uint16_t *wt, const float *params, float *pred, float *xraw,

@@ -117,9 +117,9 @@ struct Tile {
 enum StepMode {
   MODE_PLAIN = 0,    // the reference's method
   MODE_CAPTURE = 1,  // the reference's method, and write surrogate training samples
-  MODE_HYBRID = 2,   // surrogate warm start, capped polish passes, fallback flag
-  MODE_FALLBACK = 3  // the reference's method on the hybrid's fallback tiles,
-                     // keeping the surrogate's inputs for the next step
+  MODE_HYBRID = 2    // surrogate warm start, capped polish passes judged by
+                     // the leaf energy residual, the reference's iteration
+                     // where the judge rejects them
 };
 
 // globals of one step
@@ -153,10 +153,7 @@ struct StepArgs {
   unsigned long long scap;     // capacity of sx/sy
   int max_passes;            // MODE_HYBRID: dryLeaf passes per NITER pass
   float resid_max;           // MODE_HYBRID: leaf energy residual limit, W/m2
-  int *queue;                // MODE_HYBRID: tiles that fall back
-  unsigned *qlen;            // entries in queue
-  const int *tile_index;     // fallback launch: the tiles to step (or null)
-  const unsigned *ntiles;    // fallback launch: number of entries in tile_index
+  unsigned long long *nfallback;  // MODE_HYBRID: tiles that fell back (counter)
 };
 
 // literal of the working precision

@@ -30,7 +30,8 @@ EXPORTS = [
     "cable_gpu_download", "cable_gpu_set_globals", "cable_gpu_step", "cable_gpu_cbm",
     "cable_gpu_veg_cols", "cable_gpu_soil_cols", "cable_gpu_table_column",
     "cable_gpu_set_class_tables", "cable_gpu_set_forcing", "cable_gpu_run",
-    "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_counters", "cable_gpu_set_tile_order",
+    "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_last_kernel_ms", "cable_gpu_counters",
+    "cable_gpu_set_tile_order",
     "cable_gpu_run_capture", "cable_gpu_samples", "cable_gpu_copy_samples", "cable_gpu_set_surrogate",
     "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
     "cable_gpu_download_forcing", "cable_gpu_surrogate_eval", "cable_gpu_consistency_check",
@@ -85,6 +86,7 @@ def lib():
         "cable_gpu_run": ([vp, c_int, c_int, c_double], c_int),
         "cable_gpu_sync": ([vp], c_int),
         "cable_gpu_last_run_ms": ([vp, ctypes.POINTER(c_double)], c_int),
+        "cable_gpu_last_kernel_ms": ([vp, ctypes.POINTER(c_double), ctypes.POINTER(c_double)], c_int),
         "cable_gpu_counters": ([vp, ctypes.POINTER(i64)], c_int),
         "cable_gpu_last_error": ([], ctypes.c_char_p),
         "cable_gpu_set_tile_order": ([vp, vp], c_int),
@@ -345,6 +347,12 @@ class CableGPU:
         v = ctypes.c_double()
         _check(lib().cable_gpu_last_run_ms(self.h, ctypes.byref(v)))
         return v.value
+
+    def last_kernel_ms(self):
+        """(step kernels, inference kernels) device ms of the last run."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().cable_gpu_last_kernel_ms(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def sync(self):
         _check(lib().cable_gpu_sync(self.h))

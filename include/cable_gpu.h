@@ -234,6 +234,13 @@ int cable_gpu_consistency_values(cable_gpu_ctx *ctx, double *out);
    events on the context stream around its kernels. */
 int cable_gpu_last_run_ms(cable_gpu_ctx *ctx, double *ms);
 
+/* Device time of the last run's kernels, in milliseconds, summed over its
+   steps: the step kernels, and the surrogate inference kernels of a hybrid
+   run (0 otherwise); HIP events on the context stream around each launch of
+   a hybrid run, around the run otherwise.  (No counterpart in the
+   reference; a measurement.) */
+int cable_gpu_last_kernel_ms(cable_gpu_ctx *ctx, double *step_ms, double *infer_ms);
+
 /* Total dryLeaf fixed-point passes of the last step or run (summed over tiles
    and the four stability iterations), a work counter. */
 int cable_gpu_counters(cable_gpu_ctx *ctx, int64_t *dryleaf_passes);
@@ -266,18 +273,20 @@ int cable_gpu_copy_samples(cable_gpu_ctx *ctx, int64_t first, int64_t n, float *
    include/cable_mlp.h) */
 int cable_gpu_set_surrogate(cable_gpu_ctx *ctx, const float *params, const float *norm);
 /* Run steps k0..k0+nsteps-1 with the hybrid method: per step the network
-   predicts the leaf state of each sunlit leaf-bearing tile, the leaf fixed
-   point of those tiles starts there and runs at most max_passes passes per
-   NITER pass (the reference runs up to 20; every other tile runs the
-   reference's iteration), and every capped tile whose leaf energy-balance
-   residual beyond the
-   reference's 0.1 K tolerance exceeds resid_max (W/m2) is stepped again
-   with the plain method from its unchanged state.  use_surrogate = 0 runs
-   the same capped method from the reference's own starting point (the
-   comparison without the network). */
+   predicts the leaf state of each sunlit leaf-bearing tile and the leaf
+   fixed point of those tiles starts there.  After max_passes polish passes
+   (per NITER pass) such a tile is judged by its leaf energy-balance
+   residual beyond the reference's 0.1 K tolerance, cr (ghr1 + ghr2)
+   (|dT| - 0.1 K): within resid_max (W/m2) it stops; otherwise it falls back
+   to the reference's iteration, which goes on with the reference's damping
+   up to its 20 passes (src/science/canopy/cbl_dryLeaf.F90:239-606).  Every
+   other tile runs the reference's iteration.  use_surrogate = 0 runs the
+   same capped method from the reference's own starting point (the
+   comparison without the network).  Two launches per step: the inference
+   and the step. */
 int cable_gpu_run_hybrid(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, int max_passes,
                          double resid_max, int use_surrogate);
-/* fallback tile-steps and all tile-steps of the last hybrid run */
+/* tile-steps that fell back, and all tile-steps, of the last hybrid run */
 int cable_gpu_hybrid_counters(cable_gpu_ctx *ctx, int64_t *fallback_tile_steps, int64_t *tile_steps);
 /* The surrogate's view of step k from the current state, as the hybrid step
    computes it: x (mp x 24) the raw features, pred (mp x 8) the denormalised

@@ -7,7 +7,11 @@
 //
 // IN (little-endian): int64 mp, int32 nsteps, int32 full (1: every field in
 // and out, the drop-in step; 0: the resident run's compact set with the
-// class tables), int32 precision (0 FP64, 1 mixed), double dels, 18 doubles
+// class tables; 2: the resident set stepped by the hybrid method, mixed
+// precision only, from a fixed warm start -- leaf temperature tk + 0.5 K,
+// leaf-surface CO2 0.7 ca -- with one polish pass and no residual allowance,
+// so that many tiles fall back and start again), int32 precision (0 FP64,
+// 1 mixed), double dels, 18 doubles
 // zse(6) zshh(7) ratecp(3) ratecs(2), the veg and soil class tables
 // (17 x cable_gpu_veg_cols, 9 x cable_gpu_soil_cols doubles, in the column
 // order of include/cable_gpu.h), then per step: int32 ktau, int32
@@ -47,8 +51,10 @@ int dev_kind(int i) {
 }
 
 template <class R>
-int run(FILE *in, FILE *out, int64_t mp, int nsteps, bool full, double dels, const double *glob,
+int run(FILE *in, FILE *out, int64_t mp, int nsteps, int full, double dels, const double *glob,
         const std::vector<double> &vtab, const std::vector<double> &stab) {
+  if (full == 2 && sizeof(R) != 4) return 4;
+  long long fallbacks = 0;
   std::vector<std::vector<double>> h(CABLE_NFIELDS);
   std::vector<void *> dev(CABLE_NFIELDS);
   for (int i = 0; i < CABLE_NFIELDS; i++) {
@@ -68,6 +74,16 @@ int run(FILE *in, FILE *out, int64_t mp, int nsteps, bool full, double dels, con
   memcpy(a.ratecs, glob + 16, sizeof a.ratecs);
   a.vtab = vtab.data();
   a.stab = stab.data();
+  // the hybrid method's inputs (full == 2)
+  std::vector<float> pred((size_t)8 * mp, 0.0f), hyb((size_t)3 * mp, 0.0f);
+  for (int64_t t = 0; t < mp; t++) {
+    pred[8 * t + 0] = 0.5f;
+    pred[8 * t + 1] = pred[8 * t + 2] = 0.7f;
+  }
+  a.pred = pred.data();
+  a.hyb = hyb.data();
+  a.max_passes = 1;
+  a.resid_max = 0.0f;
   for (int s = 0; s < nsteps; s++) {
     int32_t ktau = 0, first = 0;
     if (!rd(in, &ktau, 4) || !rd(in, &first, 4)) return 2;
@@ -86,14 +102,19 @@ int run(FILE *in, FILE *out, int64_t mp, int nsteps, bool full, double dels, con
     a.ca = h[CABLE_F_met_ca][0];
     a.doy = h[CABLE_F_met_doy][0];
     for (int64_t t = 0; t < mp; t++) {
-      if (full) {
+      if (full == 1) {
         TileOut<MODE_PLAIN, false> o;
         step_tile<R, true, false, MODE_PLAIN, false>(a, t, nullptr, nullptr, o);
-      } else {
+      } else if (full == 0) {
         TileOut<MODE_PLAIN, false> o;
         step_tile<R, false, true, MODE_PLAIN, false>(a, t, vtab.data(), stab.data(), o);
+      } else if constexpr (sizeof(R) == 4) {
+        TileOut<MODE_HYBRID, false> o;
+        step_tile<R, false, true, MODE_HYBRID, false>(a, t, vtab.data(), stab.data(), o);
+        fallbacks += o.fallback ? 1 : 0;
       }
     }
+    a.hyb_valid = 1;
     for (int i = 0; i < CABLE_NFIELDS; i++) {
       const int k = dev_kind<R>(i);
       for (size_t j = 0; j < h[i].size(); j++) {
@@ -105,6 +126,7 @@ int run(FILE *in, FILE *out, int64_t mp, int nsteps, bool full, double dels, con
     }
   }
   for (void *p : dev) free(p);
+  if (full == 2) fprintf(stderr, "fallbacks %lld\n", fallbacks);
   return 0;
 }
 }  // namespace
@@ -125,8 +147,8 @@ int main(int argc, char **argv) {
   std::vector<double> vtab(NVEGCLASS * VC_N), stab(NSOILCLASS * SC_N);
   if (!rd(in, vtab.data(), sizeof(double) * vtab.size()) || !rd(in, stab.data(), sizeof(double) * stab.size()))
     return 2;
-  const int r = precision == 0 ? run<double>(in, out, mp, nsteps, full != 0, dels, glob, vtab, stab)
-                               : run<float>(in, out, mp, nsteps, full != 0, dels, glob, vtab, stab);
+  const int r = precision == 0 ? run<double>(in, out, mp, nsteps, full, dels, glob, vtab, stab)
+                               : run<float>(in, out, mp, nsteps, full, dels, glob, vtab, stab);
   fclose(in);
   fclose(out);
   return r;

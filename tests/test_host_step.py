@@ -70,7 +70,7 @@ def _write_input(path, wl, nsteps, full, precision, states):
     p = wl.params
     glob = np.concatenate([p["soil%zse"], p["soil%zshh"], p["bgc%ratecp"], p["bgc%ratecs"]]).astype(np.float64)
     with open(path, "wb") as f:
-        f.write(struct.pack("<qiiid", wl.n, nsteps, 1 if full else 0, precision, float(wl.dels)))
+        f.write(struct.pack("<qiiid", wl.n, nsteps, int(full), precision, float(wl.dels)))
         f.write(glob.tobytes())
         f.write(np.ascontiguousarray(V, dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(S, dtype=np.float64).tobytes())
@@ -110,6 +110,8 @@ def _oracle_steps(wl, nsteps):
 
 
 def _run(exe, tmp_path, wl, nsteps, full, precision, pre, env=None):
+    """full: True/1 the drop-in layout, False/0 the resident one, 2 the
+    hybrid method (mixed precision)."""
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     _write_input(fin, wl, nsteps, full, precision, pre)
     r = subprocess.run([exe, fin, fout], capture_output=True, text=True, env=env, timeout=600)
@@ -160,8 +162,8 @@ def test_host_build_of_the_device_step_matches_the_oracle(builds, case, tmp_path
     assert not diff, diff[:10]
 
 
-@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
-@pytest.mark.parametrize("full", [True, False], ids=["dropin", "resident"])
+@pytest.mark.parametrize("precision,full", [(0, 1), (0, 0), (1, 1), (1, 0), (1, 2)],
+                         ids=["fp64-dropin", "fp64-resident", "mixed-dropin", "mixed-resident", "hybrid"])
 def test_device_step_under_address_and_undefined_behaviour_sanitizers(builds, case, tmp_path, precision, full):
     exe = builds["asan_ubsan"]
     if not os.path.exists(exe):
@@ -169,12 +171,13 @@ def test_device_step_under_address_and_undefined_behaviour_sanitizers(builds, ca
     wl, pre, _ = case
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:halt_on_error=1",
                UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
-    _, err = _run(exe, tmp_path, wl, 12, full, precision, pre[:12], env=env)
+    steps = slice(12, 24) if full == 2 else slice(0, 12)
+    _, err = _run(exe, tmp_path, wl, 12, full, precision, pre[steps], env=env)
     assert "runtime error" not in err and "ERROR: AddressSanitizer" not in err, err[-3000:]
 
 
-@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
-@pytest.mark.parametrize("full", [True, False], ids=["dropin", "resident"])
+@pytest.mark.parametrize("precision,full", [(0, 1), (0, 0), (1, 1), (1, 0), (1, 2)],
+                         ids=["fp64-dropin", "fp64-resident", "mixed-dropin", "mixed-resident", "hybrid"])
 def test_device_step_reads_no_unset_tile_member(builds, case, tmp_path, precision, full):
     """MemorySanitizer: no branch, address or stored output of the step
     depends on a value the gather left unset (the resident layout loads only
@@ -184,5 +187,9 @@ def test_device_step_reads_no_unset_tile_member(builds, case, tmp_path, precisio
         pytest.skip("MemorySanitizer build unavailable: " + exe[-300:])
     wl, pre, _ = case
     env = dict(os.environ, MSAN_OPTIONS="halt_on_error=1")
-    _, err = _run(exe, tmp_path, wl, 12, full, precision, pre[:12], env=env)
+    # steps 12-23: daylight from 06:00, where the hybrid warm-starts and falls back
+    steps = slice(12, 24) if full == 2 else slice(0, 12)
+    _, err = _run(exe, tmp_path, wl, 12, full, precision, pre[steps], env=env)
     assert "MemorySanitizer" not in err, err[-3000:]
+    if full == 2:
+        assert int(err.split("fallbacks")[1].split()[0]) > 0, err[-500:]

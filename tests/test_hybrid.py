@@ -22,6 +22,7 @@ def _outputs(g, names=HEADLINE + ["ssnow%tgg", "ssnow%wb"]):
 
 
 def test_capture_keeps_the_plain_trajectory_and_writes_samples():
+    """The capture step writes samples and leaves the plain step's bits."""
     print("[hybrid] capture", flush=True)
     wl = Workload(8192, 16)
     g = _ctx(wl)
@@ -42,12 +43,12 @@ def test_capture_keeps_the_plain_trajectory_and_writes_samples():
     assert (y[:, 1] > 0).all() and (y[:, 2] > 0).all()
     assert np.abs(y[:, 0]).max() < 20.0
     for k in a:
-        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-2, err_msg=k)
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
 def test_uncapped_hybrid_without_network_is_the_plain_step():
     """max_passes = 20 and no network: the hybrid step is the plain method,
-    nothing falls back."""
+    bit for bit, and nothing falls back."""
     print("[hybrid] uncapped without network", flush=True)
     wl = Workload(8192, 12)
     g = _ctx(wl)
@@ -61,13 +62,13 @@ def test_uncapped_hybrid_without_network_is_the_plain_step():
     g.close()
     assert fb == 0 and ts == 8192 * 12
     for k in a:
-        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-2, err_msg=k)
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
 def test_night_steps_run_the_reference_iteration():
     """Without daylight no tile is warm-started or capped: with a network and
-    a cap of one pass, the night steps (00:00 to 05:30) queue nothing for the
-    fallback and give the plain step's fluxes."""
+    a cap of one pass, the night steps (00:00 to 05:30) send nothing to the
+    fallback and give the plain step's bits."""
     from cable_amd.surrogate import NIN, NOUT
     from oracle import mlp_oracle as O
     print("[hybrid] night steps", flush=True)
@@ -86,14 +87,14 @@ def test_night_steps_run_the_reference_iteration():
     g.close()
     assert fb == 0 and ts == 8192 * 12
     for k in a:
-        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-2, err_msg=k)
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
 def test_hybrid_is_deterministic():
-    """The fallback queue is filled by atomics in whatever order the waves
-    reach it; each tile's result does not depend on that order.  Two runs of
-    the same daylight steps, with a cap that sends tiles to the fallback,
-    give the same bits and the same counters."""
+    """The fallback count is summed by atomics in whatever order the waves
+    reach it; no tile's result depends on that order.  Two runs of the same
+    daylight steps, with a cap that sends tiles to the fallback, give the
+    same bits and the same counters."""
     from cable_amd.surrogate import NIN, NOUT
     from oracle import mlp_oracle as O
     print("[hybrid] determinism", flush=True)
