@@ -1,21 +1,26 @@
 """Throughput of the CABLE timestep (cbm) on one MI355X, in tile-steps/s.
 
 A step is one cbm pass over every tile of the synthetic workload: the
-device-resident path (cable_gpu_run) with state, parameters, class tables and
-the run's forcing already in HBM when the timed region starts.  The headline
-line is the plain FP64 path on 262,144 tiles x 48 half-hourly steps; the other
-forms -- FP64, mixed precision and the surrogate-warm-started hybrid at
-1,048,576 tiles, with the mixed and hybrid fluxes' distance from FP64 after
-the run and the surrogate's training on the GPU -- are reported beside it
-under "forms".
+device-resident path (cable_gpu_run, cable_gpu_run_hybrid) with state,
+parameters, class tables and the run's forcing already in HBM when the timed
+region starts.  The headline line is the metric's own form: the
+surrogate-warm-started hybrid step (mixed precision, BF16 network trained on
+the GPU beforehand) on 1,048,576 tiles, with the plain FP64 step on the same
+tiles beside it ("hybrid_vs_fp64") and the flux error of the hybrid and mixed
+forms against FP64 over every tile-step of the run ("gate").  The other forms
+-- plain FP64 on 262,144 tiles, mixed precision, the surrogate's training and
+the drop-in step through host buffers -- are under "forms".
 
-  python bench.py [--gpus 1] [--steps 48] [--warmup 4] [--no-forms] [--no-cpu]
+  python bench.py [--gpus 1] [--steps 48] [--warmup 4] [--form hybrid|fp64|mixed]
+                  [--tiles N] [--no-forms] [--no-cpu]
 
-Prints one JSON line.  "roofline" rates the kernel against HBM bandwidth
-using the algorithmic bytes per tile-step of the path (state read and
-written once, forcing, per-tile parameters and the eight outputs once);
-"cpu_baseline" times the reference's own cbm (built from its sources) on a
-prefix of the same workload on one host core.
+Prints one JSON line.  "roofline" rates the dominant kernel (the step kernel
+of the headline form) against HBM bandwidth with the algorithmic bytes per
+tile-step of the path (state read and written once, forcing, per-tile
+parameters and the eight outputs once); the kernel is bound by vector-ALU
+issue, not by HBM ("bound": "valu", figures from profiles/).
+"cpu_baseline" times the reference's own cbm (built from its sources) on the
+GPU box's host cores.
 """
 from __future__ import annotations
 
@@ -30,21 +35,51 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "tile-steps/sec on 1 MI355X: hybrid vs plain FP64, at ≤1 W/m² flux error"
+METRIC = "tile-steps/sec on 1 MI355X: hybrid vs plain FP64, at \u22641 W/m\u00b2 flux error"
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per tile-step: prognostic state read and written once,
 # 8 forcing fields and the per-tile parameters read once, 8 outputs written
+# (the hybrid also reads and writes the previous step's leaf state)
 ALGO_BYTES = {"fp64": 1016, "mixed": 756, "hybrid": 764}
-CPU_SAMPLE_TILES = 49152
-
+# the reference sample timed on the host: the first tiles of the headline
+# workload, split over the host cores
+CPU_SAMPLE_TILES = 65536
+# the box's CPU share for one GPU
+CPU_SHARE = 16
 
 HEADLINE = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes", "canopy%fgpp",
             "ssnow%runoff", "rad%trad"]
-
+# the gate: 1 W/m2 for the energy fluxes, 0.1 umol CO2 m-2 s-1 for carbon
+# (canopy%fgpp is in g C m-2 s-1: 1 umol = 12e-6 g C)
+GATE_FLUXES = ["canopy%rnet", "canopy%fe", "canopy%fh", "canopy%fev", "canopy%fes", "canopy%fgpp"]
+GC_PER_UMOL = 12.0e-6
+GATE = dict({k: 1.0 for k in GATE_FLUXES[:5]}, **{"canopy%fgpp": 0.1})
+# the reference's own spread over daylight leaf-bearing tile-steps, measured
+# on 49,152 tiles x 48 steps of this workload (reference built from its
+# sources, oracle/spread.py): p99.9 and fraction beyond the gate, for its
+# shipped build against the promoted one and for a 0.1 K and a 1 K change of
+# its leaf temperature start
+REFERENCE_SPREAD = {
+    "shipped_vs_promoted": {"canopy%rnet": (0.0059, 2.0e-5), "canopy%fe": (0.043, 1.3e-4),
+                            "canopy%fh": (0.033, 1.3e-4), "canopy%fev": (0.029, 8.6e-5),
+                            "canopy%fes": (0.012, 5.7e-5), "canopy%fgpp": (0.00063, 3.5e-5)},
+    "leaf_start_0.1K": {"canopy%rnet": (4.83, 3.7e-3), "canopy%fe": (16.3, 1.48e-2),
+                        "canopy%fh": (11.9, 1.26e-2), "canopy%fev": (11.1, 1.22e-2),
+                        "canopy%fes": (7.62, 5.3e-3), "canopy%fgpp": (0.857, 4.3e-3)},
+    "leaf_start_1K": {"canopy%rnet": (4.41, 3.24e-3), "canopy%fe": (15.3, 1.42e-2),
+                      "canopy%fh": (11.7, 1.21e-2), "canopy%fev": (10.6, 1.15e-2),
+                      "canopy%fes": (6.87, 4.9e-3), "canopy%fgpp": (1.013, 4.45e-3)},
+}
 
 # compared between forms: the headline fluxes, soil temperature (K) and soil
 # moisture (m3/m3) of all six layers
 COMPARED = HEADLINE + ["ssnow%tgg", "ssnow%wb"]
+
+TRAIN = {"n_tiles": 262144, "nsteps": 96, "epochs": 2.0, "max_samples": 16777216}
+# the hybrid's polish passes per NITER pass and its leaf energy residual limit
+# (W/m2): after HYB_CAP passes a warm-started tile whose residual exceeds
+# HYB_RESID falls back to the reference's iteration
+HYB_CAP, HYB_RESID = 4, 0.5
 
 
 def flux_errors(got, ref):
@@ -77,7 +112,13 @@ def class_costs(wl, prec, log):
 ORDER_LABEL = "grouped by vegetation type (measured most expensive first), then soil type and leaf area"
 
 
-def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None, keep=False):
+ORDER_LABEL = "grouped by vegetation type (measured most expensive first), then soil type and leaf area"
+
+
+def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None):
+    """Time ``steps`` resident steps of ``precision`` ("fp64", "mixed"; with
+    ``hybrid``, the hybrid method) on the n-tile workload, after ``warmup``
+    untimed steps in another context of the same workload."""
     from cable_amd import gpu
     from cable_amd.workload import Workload
     nsteps = max(steps, warmup, 1)
@@ -87,38 +128,100 @@ def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None, ke
     prec = gpu.PREC_FP64 if precision == "fp64" else gpu.PREC_MIXED
     order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(wl, prec, log)) if class_order else None
 
+    def ctx():
+        g = gpu.from_workload(wl, precision=prec, full=False, order=order)
+        if hybrid is not None:
+            g.set_surrogate(hybrid["params"], hybrid["norm"])
+        g.sync()
+        return g
+
     def go(g, k):
         if hybrid is None:
             g.run(0, k, wl.dels)
         else:
-            g.set_surrogate(hybrid["params"], hybrid["norm"])
             g.run_hybrid(0, k, wl.dels, hybrid["max_passes"], hybrid["resid_max"])
     if warmup:
-        g = gpu.from_workload(wl, precision=prec, full=False, order=order)
+        g = ctx()
         go(g, warmup)
         g.sync()
         g.close()
-    g = gpu.from_workload(wl, precision=prec, full=False, order=order)
-    g.sync()
+    g = ctx()
     t0 = time.perf_counter()
     go(g, steps)
     g.sync()
     wall = time.perf_counter() - t0
     ev_ms = g.last_run_ms()
+    step_ms, infer_ms = g.last_kernel_ms()
     passes = g.dryleaf_passes()
     extra = {}
     if hybrid is not None:
         fb, ts = g.hybrid_counters()
         extra["fallback_fraction"] = fb / ts
-    if keep:
-        extra["_out"] = {k: g.download(k) for k in COMPARED}
+        extra["inference_ms"] = infer_ms / steps
     g.close()
-    per_launch_s = ev_ms * 1e-3 / steps
-    achieved = ALGO_BYTES[precision] * n / per_launch_s / 1e9
-    log(f"[bench] {precision} n={n}: wall {wall * 1e3:.2f} ms for {steps} steps, "
-        f"events {ev_ms:.2f} ms, {n * steps / wall:.4g} tile-steps/s")
+    form = "hybrid" if hybrid is not None else precision
+    achieved = ALGO_BYTES[form] * n / (step_ms * 1e-3 / steps) / 1e9
+    log(f"[bench] {form} n={n}: wall {wall * 1e3:.2f} ms for {steps} steps, "
+        f"events {ev_ms:.2f} ms (step kernels {step_ms:.2f}), {n * steps / wall:.4g} tile-steps/s")
     return dict({"tiles": n, "steps": steps, "value": n * steps / wall, "ms_per_step": wall * 1e3 / steps,
-                 "kernel_ms": ev_ms / steps, "achieved_GBs": achieved, "dryleaf_passes": passes}, **extra)
+                 "device_ms_per_step": ev_ms / steps, "step_kernel_ms": step_ms / steps,
+                 "achieved_GBs": achieved, "dryleaf_passes": passes}, **extra)
+
+
+def _quantile(d, q):
+    k = min(d.size - 1, int(np.ceil(q * d.size)) - 1)
+    return float(np.partition(d, k)[k]) if d.size else 0.0
+
+
+def gate_stats(n, steps, params, norm, log):
+    """Flux error of the mixed and hybrid forms against plain FP64 at every
+    tile-step of an n-tile, ``steps``-step run (untimed; the three forms are
+    stepped side by side and downloaded after each step).  Energy fluxes in
+    W/m2, canopy%fgpp in umol CO2 m-2 s-1; over every tile-step and over the
+    daylight leaf-bearing ones (those the hybrid warm-starts)."""
+    from cable_amd import gpu
+    from cable_amd.workload import Workload
+    t = time.perf_counter()
+    wl = Workload(n, steps)
+    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai)
+    ctx = {"fp64": gpu.from_workload(wl, precision=gpu.PREC_FP64, full=False, order=order),
+           "mixed": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order),
+           "hybrid": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order)}
+    ctx["hybrid"].set_surrogate(params, norm)
+    diffs = {f: {k: [] for k in GATE_FLUXES} for f in ("mixed", "hybrid")}
+    fsd = wl.forcing["met%fsd"][:, 0, :]
+    day = (2.0 * fsd > 0.001) & (wl.iveg < 14)[None, :]
+    for s in range(steps):
+        ctx["fp64"].run(s, 1, wl.dels)
+        ctx["mixed"].run(s, 1, wl.dels)
+        ctx["hybrid"].run_hybrid(s, 1, wl.dels, HYB_CAP, HYB_RESID)
+        ref = {k: ctx["fp64"].download(k).reshape(-1) for k in GATE_FLUXES}
+        for f in ("mixed", "hybrid"):
+            for k in GATE_FLUXES:
+                d = np.abs(ctx[f].download(k).reshape(-1).astype(np.float64) - ref[k])
+                if k == "canopy%fgpp":
+                    d = d / GC_PER_UMOL
+                diffs[f][k].append(d.astype(np.float32))
+    for g in ctx.values():
+        g.close()
+    out = {"tiles": n, "steps": steps, "units": "W/m2; canopy%fgpp in umol CO2 m-2 s-1",
+           "thresholds": GATE, "reference_spread_daylight_leafy": REFERENCE_SPREAD}
+    for f in ("mixed", "hybrid"):
+        out[f] = {}
+        for sub, mask in (("all", None), ("daylight_leafy", day)):
+            r = {}
+            for k in GATE_FLUXES:
+                d = np.stack(diffs[f][k])
+                d = (d if mask is None else d[mask]).ravel()
+                r[k] = {"max": float(d.max()), "p99.9": _quantile(d, 0.999), "p99": _quantile(d, 0.99),
+                        "frac_beyond": float((d > GATE[k]).mean())}
+            out[f][sub] = r
+        # the gate as stated (every flux on every tile-step) and at the 99.9th
+        # percentile of the daylight leaf-bearing tile-steps
+        out[f]["every_value_within_gate"] = all(out[f]["all"][k]["max"] <= GATE[k] for k in GATE_FLUXES)
+        out[f]["p99.9_within_gate"] = all(out[f]["daylight_leafy"][k]["p99.9"] <= GATE[k] for k in GATE_FLUXES)
+    log(f"[bench] gate statistics over {n} tiles x {steps} steps in {time.perf_counter() - t:.1f}s")
+    return out
 
 
 def dropin_form(n, steps, log):
@@ -162,73 +265,81 @@ def dropin_form(n, steps, log):
             "upload_ms": up * 1e3 / steps, "step_ms": dev * 1e3 / steps, "download_ms": dn * 1e3 / steps}
 
 
-def profiled_traffic(tiles, precision):
-    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
-    scaled by a plain copy) for this workload, from the committed profile
-    summary of the same kernel, or None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"cbm_kernel_{precision}_{tiles}x48.md")
-    if not os.path.exists(path):
-        return None, None
-    with open(path) as fh:
-        for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
-    return None, None
-
-
-def cpu_baseline(steps, log):
-    """The reference's cbm (oracle/_ref, compiled from its sources) on the
-    first CPU_SAMPLE_TILES tiles of the workload, one process on one core;
-    the CPU restatement (oracle/) when the reference build is absent."""
-    from cable_amd.workload import Workload
-    wl = Workload(CPU_SAMPLE_TILES, steps)
-    try:
-        from oracle import refrun
-        if refrun.available("r8"):
-            _, sec = refrun.run(wl, steps, [], every_step=False, build="r8")
-            kind, what = "reference", "reference cbm promoted to FP64 (amdflang -O2), driver step loop"
-        else:
-            raise FileNotFoundError
-    except (FileNotFoundError, OSError):
-        from oracle.oracle import OracleRun
-        _, sec = OracleRun(wl).run(nsteps=steps, outputs=[], every_step=False)
-        kind, what = "port", "FP64 C restatement of the reference cbm (gcc -O2)"
-    v = CPU_SAMPLE_TILES * steps / sec
-    log(f"[bench] cpu baseline ({kind}): {sec:.2f}s, {v:.4g} tile-steps/s")
-    out = {"value": v, "unit": "tile-steps/s", "cores": 1, "kind": kind,
-           "sample": f"first {CPU_SAMPLE_TILES} tiles of the workload x {steps} steps; {what}; "
-                     f"one process; {sec:.2f} s"}
-    if kind == "reference":
-        out.update(_cpu_reference_more(wl, steps, log))
+def profiled(name):
+    """Figures of the committed profile summary profiles/<name>.md (lines
+    'key: value'), or {} when there is none."""
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    out = {}
+    if os.path.exists(path):
+        with open(path) as fh:
+            for line in fh:
+                k, sep, v = line.partition(":")
+                if sep and k.strip() in ("traffic_bytes_per_launch", "valu_issue_utilisation", "fp32_valu_frac",
+                                         "fp64_valu_frac", "mfma_busy_frac"):
+                    try:
+                        out[k.strip()] = float(v)
+                    except ValueError:
+                        pass
+        out["source"] = os.path.relpath(path, ROOT)
     return out
 
 
-def _cpu_reference_more(wl, steps, log):
-    """The shipped-precision reference on one core, and the promoted one on
-    all host cores (one process per contiguous block of tiles, as the MPI
-    workers split land points; cbm keeps module state, so no threads)."""
+def cpu_baseline(steps, log):
+    """The reference's cbm (oracle/_ref/r8, promoted to FP64, compiled from
+    its sources) on the host: the first CPU_SAMPLE_TILES tiles of the
+    workload, one process per host core of the box's share, each pinned to
+    its core and owning a contiguous block of tiles (as the MPI workers
+    split land points, src/offline/cable_mpimaster.F90:1399-1487; cbm keeps
+    module state, so no threads); the reference's own configuration 0
+    (4,096 tiles x 48 steps) the same way; one core alone, promoted and
+    shipped precision.  The CPU restatement (oracle/) stands in when the
+    reference build is absent."""
     import concurrent.futures as cf
+
+    from cable_amd.workload import Workload
     from oracle import refrun
-    more = {}
+    cores = sorted(os.sched_getaffinity(0))[:CPU_SHARE]
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), model)
+    except OSError:
+        pass
+    host = {"cpu_model": model, "host_cores": os.cpu_count(), "cores_used": len(cores)}
+    if not refrun.available("r8"):
+        from oracle.oracle import OracleRun
+        wl = Workload(4096, steps)
+        _, sec = OracleRun(wl).run(nsteps=steps, outputs=[], every_step=False)
+        return dict(host, value=4096 * steps / sec, unit="tile-steps/s", cores=1, kind="port",
+                    sample=f"first 4096 tiles x {steps} steps, FP64 C restatement of the reference cbm, one core")
+
+    def spread(wl, build="r8"):
+        blocks = np.array_split(np.arange(wl.n), len(cores))
+        with cf.ThreadPoolExecutor(len(cores)) as ex:
+            secs = list(ex.map(lambda bc: refrun.run(wl.subset(bc[0]), wl.nsteps, [], every_step=False, build=build,
+                                                     cpu=bc[1])[1], zip(blocks, cores)))
+        return max(secs)
+
+    wl = Workload(CPU_SAMPLE_TILES, steps)
+    sec = spread(wl)
+    v = CPU_SAMPLE_TILES * steps / sec
+    log(f"[bench] cpu baseline: reference on {len(cores)} pinned processes, {sec:.2f}s, {v:.4g} tile-steps/s")
+    out = dict(host, value=v, unit="tile-steps/s", cores=len(cores), kind="reference",
+               sample=f"first {CPU_SAMPLE_TILES} tiles of the workload x {steps} steps; reference cbm promoted to "
+                      f"FP64 (amdflang -O2), driver step loop; {len(cores)} processes of contiguous tile blocks, "
+                      f"each pinned to one core ({model}); slowest process {sec:.2f} s")
+    cfg0 = Workload(4096, 48)
+    s0 = spread(cfg0)
+    out["cfg0_4096x48"] = {"value": 4096 * 48 / s0, "seconds": s0, "processes": len(cores)}
+    log(f"[bench] cpu baseline cfg 0 (4096 x 48): {s0:.3f}s on {len(cores)} processes")
+    one = Workload(16384, steps)
+    _, s1 = refrun.run(one, steps, [], every_step=False, build="r8", cpu=cores[0])
+    out["one_core"] = {"value": 16384 * steps / s1, "seconds": s1, "sample": f"16384 tiles x {steps} steps"}
     if refrun.available("r4"):
-        _, sec = refrun.run(wl, steps, [], every_step=False, build="r4")
-        more["shipped_precision_one_core"] = {"value": CPU_SAMPLE_TILES * steps / sec, "seconds": sec}
-        log(f"[bench] cpu baseline shipped precision: {sec:.2f}s")
-    p = max(1, min(16, os.cpu_count() or 1))
-    blocks = np.array_split(np.arange(CPU_SAMPLE_TILES), p)
-    with cf.ThreadPoolExecutor(p) as ex:
-        secs = list(ex.map(lambda b: refrun.run(wl.subset(b), steps, [], every_step=False, build="r8")[1], blocks))
-    more["all_cores"] = {"value": CPU_SAMPLE_TILES * steps / max(secs), "processes": p,
-                         "seconds": max(secs)}
-    log(f"[bench] cpu baseline on {p} processes: {max(secs):.2f}s")
-    return more
-
-
-TRAIN = {"n_tiles": 262144, "nsteps": 96, "epochs": 2.0, "max_samples": 16777216}
-# the hybrid's polish passes per NITER pass and its leaf energy residual limit
-# (W/m2).  Over the week-long run a cap of 3 was more accurate but slower
-# than 4 (5.5% of tile-steps fall back instead of 0.4%)
-HYB_CAP, HYB_RESID = 4, 0.5
+        _, s4 = refrun.run(one, steps, [], every_step=False, build="r4", cpu=cores[0])
+        out["one_core_shipped_precision"] = {"value": 16384 * steps / s4, "seconds": s4}
+    log(f"[bench] cpu baseline one core: promoted {s1:.2f}s" + (f", shipped {s4:.2f}s" if refrun.available("r4") else ""))
+    return out
 
 
 def production(args, log):
@@ -252,6 +363,7 @@ def production(args, log):
     ctx["hybrid"].set_surrogate(params, norm)
     wall = {k: 0.0 for k in ctx}
     dev_ms = {k: 0.0 for k in ctx}
+    step_ms = {k: 0.0 for k in ctx}
     passes = {k: 0 for k in ctx}
     fb = ts = 0
     synth_s = 0.0
@@ -275,6 +387,7 @@ def production(args, log):
                 g.sync()
                 wall[name] += time.perf_counter() - t
                 dev_ms[name] += g.last_run_ms()
+                step_ms[name] += g.last_kernel_ms()[0]
                 passes[name] += g.dryleaf_passes()
                 if name == "hybrid":
                     f, s_ = g.hybrid_counters()
@@ -288,7 +401,8 @@ def production(args, log):
     forms = {}
     for k in ctx:
         forms[f"{k}_{n}"] = {"tiles": n, "steps": total, "value": n * total / wall[k],
-                             "ms_per_step": wall[k] * 1e3 / total, "kernel_ms": dev_ms[k] / total,
+                             "ms_per_step": wall[k] * 1e3 / total, "device_ms_per_step": dev_ms[k] / total,
+                             "step_kernel_ms": step_ms[k] / total,
                              "dryleaf_passes": passes[k]}
     forms[f"mixed_{n}"]["flux_error_vs_fp64_after_run"] = flux_errors(out["mixed"], out["fp64"])
     forms[f"hybrid_{n}"]["flux_error_vs_fp64_after_run"] = flux_errors(out["hybrid"], out["fp64"])
@@ -302,7 +416,7 @@ def production(args, log):
     forms["mixed_vs_fp64"] = wall["fp64"] / wall["mixed"]
     forms["forcing_synthesis_s"] = synth_s
     hyb = forms[f"hybrid_{n}"]
-    achieved = ALGO_BYTES["hybrid"] * n / (hyb["kernel_ms"] * 1e-3) / 1e9
+    achieved = ALGO_BYTES["hybrid"] * n / (hyb["step_kernel_ms"] * 1e-3) / 1e9
     return {
         "metric": METRIC, "value": hyb["value"], "unit": "tile-steps/s", "n_gpus": 1, "steps": total,
         "warmup": 0, "ms_per_step": hyb["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -311,7 +425,7 @@ def production(args, log):
         "config": {"workload": f"hybrid production run, {n} tiles x {total} half-hourly steps",
                    "tiles": n, "precision": "mixed + BF16 surrogate warm start", "path": "device-resident run",
                    "tile_order": ORDER_LABEL, "forcing_window_steps": win},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,
         "forms": forms,
@@ -323,10 +437,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--tiles", type=int, default=262144)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                    help="path of the headline line (fp64: the plain FP64 step)")
-    ap.add_argument("--no-forms", action="store_true")
+    ap.add_argument("--form", choices=["hybrid", "fp64", "mixed"], default="hybrid",
+                    help="form of the headline line (hybrid: the metric's own form)")
+    ap.add_argument("--tiles", type=int, default=None,
+                    help="tiles of the headline form (default 1,048,576; the production run 4,194,304)")
+    ap.add_argument("--no-forms", action="store_true", help="only the headline form (for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--caller-order", action="store_true",
                     help="keep the tiles in the caller's order on the device (no class grouping)")
@@ -357,42 +472,39 @@ def main():
         print(msg, file=sys.stderr, flush=True)
 
     if args.production:
-        if args.tiles == 262144:
-            args.tiles = 4194304
+        args.tiles = args.tiles or 4194304
         if args.steps == 48:
             args.steps = 336
         print(json.dumps(production(args, log)), flush=True)
         return
+    n = args.tiles or 1048576
+    co = not args.caller_order
     # the CPU leg first, while this process has not touched the GPU
     cpu = None if args.no_cpu else cpu_baseline(args.steps, log)
-    co = not args.caller_order
-    prec = args.precision
-    head = run_form(args.tiles, args.steps, args.warmup, prec, log, co)
-    forms = {f"{prec}_{args.tiles}": head}
-    if not args.no_forms:
+    forms = {}
+    if args.form == "hybrid":
         from cable_amd import surrogate
-        f64 = run_form(1048576, args.steps, args.warmup, "fp64", log, co, keep=True)
-        mix = run_form(1048576, args.steps, args.warmup, "mixed", log, co, keep=True)
-        # the surrogate: samples from the plain mixed kernel on a fresh workload
-        # of the same distribution, trained by the GPU kernels
+        # the surrogate: samples from the plain mixed kernel on a fresh
+        # workload of the same distribution, trained by the GPU kernels
         params, norm, info = surrogate.train_on_gpu(log=log, **TRAIN)
         flops = 3 * 2 * (24 * 128 + 2 * 128 * 128 + 128 * 8) * info["batches"] * 2048
         info["training_TFLOPs"] = flops / info["train_s"] / 1e12
-        hyb = run_form(1048576, args.steps, args.warmup, "mixed", log, co, keep=True,
-                       hybrid={"params": params, "norm": norm, "max_passes": HYB_CAP, "resid_max": HYB_RESID})
-        ref = f64.pop("_out")
-        mix["flux_error_vs_fp64_after_run"] = flux_errors(mix.pop("_out"), ref)
-        hyb["flux_error_vs_fp64_after_run"] = flux_errors(hyb.pop("_out"), ref)
-        hyb["surrogate_training"] = info
-        hyb["max_passes"], hyb["resid_max_Wm2"] = HYB_CAP, HYB_RESID
+        hyb = {"params": params, "norm": norm, "max_passes": HYB_CAP, "resid_max": HYB_RESID}
+        head = run_form(n, args.steps, args.warmup, "mixed", log, co, hybrid=hyb)
+        head.update(max_passes=HYB_CAP, resid_max_Wm2=HYB_RESID, surrogate_training=info)
+        forms[f"hybrid_{n}"] = head
+        f64 = run_form(n, args.steps, args.warmup, "fp64", log, co)
+        forms[f"fp64_{n}"] = f64
+    else:
+        head = run_form(n, args.steps, args.warmup, args.form, log, co)
+        forms[f"{args.form}_{n}"] = head
+    if not args.no_forms:
+        if args.form == "hybrid":
+            mix = run_form(n, args.steps, args.warmup, "mixed", log, co)
+            forms[f"mixed_{n}"] = mix
+            forms[f"mixed_vs_fp64_at_{n}"] = mix["value"] / f64["value"]
+        forms["fp64_262144"] = run_form(262144, args.steps, args.warmup, "fp64", log, co)
         forms["dropin_fp64_262144"] = dropin_form(262144, 3, log)
-        forms["fp64_1048576"] = f64
-        forms["mixed_1048576"] = mix
-        forms["hybrid_1048576"] = hyb
-        forms["mixed_vs_fp64_at_1048576"] = mix["value"] / f64["value"]
-        forms["hybrid_vs_fp64_at_1048576"] = hyb["value"] / f64["value"]
-    traffic, traffic_src = profiled_traffic(args.tiles, prec) if not args.caller_order else (None, None)
-    what = {"fp64": "cbm plain FP64", "mixed": "cbm mixed precision (FP32 canopy, FP64 solves and balances)"}[prec]
     out = {
         "metric": METRIC,
         "value": head["value"],
@@ -404,17 +516,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == "fp64" else "f32+f64",
+    }
+    if args.form == "hybrid":
+        out["hybrid_vs_fp64"] = head["value"] / f64["value"]
+        out["gate"] = gate_stats(n, args.steps, params, norm, log)
+        what = (f"hybrid cbm step (mixed precision, BF16 surrogate warm start of the leaf fixed point, "
+                f"{HYB_CAP} polish passes, fallback above {HYB_RESID} W/m2), {n} tiles x {args.steps} "
+                f"half-hourly steps from midnight, plain FP64 on the same tiles beside it")
+        dtype, kern = "bf16+f32+f64", "cbm_kernel_hybrid"
+    else:
+        what = {"fp64": "cbm plain FP64", "mixed": "cbm mixed precision (FP32 canopy, FP64 solves and balances)"}[
+            args.form] + f", {n} tiles x {args.steps} half-hourly steps"
+        dtype, kern = ("f64" if args.form == "fp64" else "f32+f64"), f"cbm_kernel_{args.form}"
+    prof = profiled(f"{kern}_{n}x{args.steps}") if co else {}
+    out.update({
+        "dtype": dtype,
         "data": "synthetic (seeded forcing, parameters from the reference's PFT and soil tables)",
-        "config": {"workload": f"{what}, {args.tiles} tiles x {args.steps} half-hourly steps",
-                   "tiles": args.tiles, "precision": prec, "path": "device-resident run",
+        "config": {"workload": what, "tiles": n, "form": args.form, "path": "device-resident run",
                    "tile_order": "caller" if args.caller_order else ORDER_LABEL},
-        "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": traffic_src},
+        "roofline": {"bound": "valu", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": prof.get("traffic_bytes_per_launch"),
+                     "kernel": "step kernel of the headline form (per-launch HIP events on the context stream)",
+                     "valu": {k: prof[k] for k in ("valu_issue_utilisation", "fp32_valu_frac", "fp64_valu_frac")
+                              if k in prof},
+                     "profile": prof.get("source")},
         "cpu_baseline": cpu,
         "forms": forms,
-    }
+    })
     print(json.dumps(out), flush=True)
 
 

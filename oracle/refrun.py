@@ -46,13 +46,14 @@ def _set(f, name, arr):
 
 
 def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
-        extra_sets=None, timeout=3600, poison=None):
+        extra_sets=None, timeout=3600, poison=None, cpu=None):
     """Run ``nsteps`` steps of the reference on workload ``wl``.
 
     outputs: field names to read back; with every_step they are read after
     each step (result[name] has a leading step axis), else after the last.
     poison: optional (step, name, value) -- overwrite a field before that
     step (0-based), used to find which fields carry state between steps.
+    cpu: optional host core to pin the driver process to (taskset).
     Returns (dict, seconds the driver spent in its step operations -- cbm
     and the driver contract around it -- measured inside the driver process,
     without its file input and output).
@@ -85,7 +86,8 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
         # the reference keeps its per-step temporaries on the stack (automatic
         # arrays of mp elements), so a large mp needs the stack limit raised,
         # as the offline model is run with `ulimit -s unlimited`
-        subprocess.run([driver_path(build), pin, pout], check=True,
+        pin_cmd = ["taskset", "-c", str(cpu)] if cpu is not None else []
+        subprocess.run(pin_cmd + [driver_path(build), pin, pout], check=True,
                        timeout=timeout, stdout=subprocess.DEVNULL, preexec_fn=_big_stack)
         dt = time.perf_counter() - t0
         res = {}
