@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/cable_gpu.h"
@@ -67,6 +68,33 @@ size_t dev_elem(const FieldMeta &f, int precision) {
     case 'D': case 'B': return 8;
     default: return precision == CABLE_PREC_MIXED ? 4 : 8;
   }
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The hybrid run's scratch for cable_gpu_run_hybrid_noalloc: the network's
+// predictions of a step (NOUT floats per tile) and two counters, in a device
+// array of the code object, so that the run allocates nothing.  Every call
+// on a GPU shares that GPU's copy; calls that overlap on one GPU race.
+
+namespace {
+constexpr int64_t NOALLOC_MAX_TILES = 1048576;
+constexpr size_t NOALLOC_PRED_BYTES = sizeof(float) * cbm_mlp::NOUT * (size_t)NOALLOC_MAX_TILES;
+constexpr size_t NOALLOC_BYTES = NOALLOC_PRED_BYTES + 256;
+constexpr int MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+// the device address of `scratch` on each device (the host-side name is a
+// stand-in, not a device address)
+void *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// counters to zero with a kernel (a fill node, not a memset, under capture)
+__global__ void zero_counters(unsigned long long *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 }  // namespace
 
@@ -228,6 +256,20 @@ int cable_gpu_create(cable_gpu_ctx **out, int64_t mp, int precision) {
   c->precision = precision;
   c->dev.assign(CABLE_NFIELDS, nullptr);
   hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess && (c->device < 0 || c->device >= MAX_DEVICES)) {
+    delete c;
+    return fail(CABLE_E_ARG, "cable_gpu_create: device %d beyond the %d supported", c->device, MAX_DEVICES);
+  }
+  if (e == hipSuccess) {
+    // the scratch array's address on this device, once per device, with the
+    // device current
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[c->device]) {
+      void *p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e == hipSuccess) g_scratch[c->device] = p;
+    }
+  }
   if (e == hipSuccess) e = hipStreamCreate(&c->stream);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -1171,6 +1213,10 @@ int cable_gpu_set_surrogate(cable_gpu_ctx *c, const float *params, const float *
   HIPCHECK(hipMemcpy(c->sparams, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c->snorm, norm, sizeof(float) * NNORM, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c->swt, wt.data(), sizeof(uint16_t) * wt.size(), hipMemcpyHostToDevice));
+  // the previous step's leaf state the network reads (also kept by the
+  // allocation-free hybrid run)
+  int r = ensure_hyb(c);
+  if (r) return r;
   return CABLE_OK;
 }
 
@@ -1236,6 +1282,57 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
   }
   c->last_passes = (int64_t)np;
   c->last_fallbacks = (int64_t)fb;
+  c->last_tile_steps = (int64_t)nsteps * c->mp;
+  return CABLE_OK;
+}
+
+int cable_gpu_run_hybrid_noalloc(cable_gpu_ctx *c, int k0, int nsteps, double dels, int max_passes,
+                                 double resid_max, int use_surrogate) {
+  int r = check_resident(c, k0, nsteps, dels, "cable_gpu_run_hybrid_noalloc");
+  if (r) return r;
+  if (c->mp > NOALLOC_MAX_TILES)
+    return fail(CABLE_E_ARG, "cable_gpu_run_hybrid_noalloc: %lld tiles, at most %lld (the size of its scratch)",
+                (long long)c->mp, (long long)NOALLOC_MAX_TILES);
+  if (use_surrogate && !c->swt) return fail(CABLE_E_STATE, "cable_gpu_run_hybrid_noalloc: no surrogate set");
+  if (!c->hyb)
+    return fail(CABLE_E_STATE, "cable_gpu_run_hybrid_noalloc: no surrogate state (call cable_gpu_set_surrogate)");
+  if (max_passes < 1 || max_passes > 20)
+    return fail(CABLE_E_ARG, "cable_gpu_run_hybrid_noalloc: max_passes %d", max_passes);
+  void *base = g_scratch[c->device];
+  if (!base) return fail(CABLE_E_STATE, "cable_gpu_run_hybrid_noalloc: no scratch on device %d", c->device);
+  float *pred = (float *)base;
+  unsigned long long *cnt = (unsigned long long *)((char *)base + NOALLOC_PRED_BYTES);  // passes, fallbacks
+  hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, c->stream, cnt, 2);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  for (int k = k0; k < k0 + nsteps; k++) {
+    StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)(k - c->forc_k0) * c->mp);
+    a.npass = cnt;
+    a.hyb = c->hyb;
+    a.hyb_valid = c->hyb_valid ? 1 : 0;
+    a.norm = c->snorm;
+    a.pred = use_surrogate ? pred : nullptr;
+    a.max_passes = max_passes;
+    a.resid_max = (float)resid_max;
+    a.nfallback = cnt + 1;
+    hipError_t e = hipSuccess;
+    if (use_surrogate) e = launch_mlp_infer(a, c->swt, c->sparams, pred, nullptr, c->mp, c->stream);
+    if (e == hipSuccess) e = launch_mixed_mode(MODE_HYBRID, a, c->mp, c->stream);
+    if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run_hybrid_noalloc launch: %s", hipGetErrorString(e));
+    c->first_call = false;
+    c->hyb_valid = true;
+  }
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  HIPCHECK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  unsigned long long h[2] = {0ull, 0ull};
+  HIPCHECK(hipMemcpy(h, cnt, sizeof h, hipMemcpyDeviceToHost));
+  c->last_ms = ms;
+  c->last_step_ms = ms;
+  c->last_infer_ms = 0.0;
+  c->last_passes = (int64_t)h[0];
+  c->last_fallbacks = (int64_t)h[1];
   c->last_tile_steps = (int64_t)nsteps * c->mp;
   return CABLE_OK;
 }

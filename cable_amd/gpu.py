@@ -33,7 +33,7 @@ EXPORTS = [
     "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_last_kernel_ms", "cable_gpu_counters",
     "cable_gpu_set_tile_order",
     "cable_gpu_run_capture", "cable_gpu_samples", "cable_gpu_copy_samples", "cable_gpu_set_surrogate",
-    "cable_gpu_run_hybrid", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
+    "cable_gpu_run_hybrid", "cable_gpu_run_hybrid_noalloc", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
     "cable_gpu_download_forcing", "cable_gpu_surrogate_eval", "cable_gpu_consistency_check",
     "cable_gpu_consistency_values", "cable_gpu_state_bytes", "cable_gpu_save_state", "cable_gpu_load_state",
     "cable_gpu_set_met", "cable_gpu_met_clock",
@@ -95,6 +95,7 @@ def lib():
         "cable_gpu_copy_samples": ([vp, i64, i64, vp, vp], c_int),
         "cable_gpu_set_surrogate": ([vp, vp, vp], c_int),
         "cable_gpu_run_hybrid": ([vp, c_int, c_int, c_double, c_int, c_double, c_int], c_int),
+        "cable_gpu_run_hybrid_noalloc": ([vp, c_int, c_int, c_double, c_int, c_double, c_int], c_int),
         "cable_gpu_hybrid_counters": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], c_int),
         "cable_gpu_synth_forcing": ([vp, c_int, c_int, ctypes.c_uint64, c_double, c_double, c_double, vp, vp],
                                     c_int),
@@ -328,6 +329,13 @@ class CableGPU:
     def run_hybrid(self, k0: int, nsteps: int, dels: float, max_passes=2, resid_max=0.5, use_surrogate=True):
         _check(lib().cable_gpu_run_hybrid(self.h, int(k0), int(nsteps), float(dels), int(max_passes),
                                           float(resid_max), 1 if use_surrogate else 0))
+
+    def run_hybrid_noalloc(self, k0: int, nsteps: int, dels: float, max_passes=2, resid_max=0.5,
+                           use_surrogate=True):
+        """run_hybrid through the entry point that allocates nothing (at most
+        1,048,576 tiles; the surrogate must be set first)."""
+        _check(lib().cable_gpu_run_hybrid_noalloc(self.h, k0, nsteps, dels, max_passes, resid_max,
+                                                  1 if use_surrogate else 0))
 
     def surrogate_eval(self, k: int):
         """(raw features (mp, 24), predictions (mp, 8)) of step k from the

@@ -286,6 +286,23 @@ int cable_gpu_set_surrogate(cable_gpu_ctx *ctx, const float *params, const float
    and the step. */
 int cable_gpu_run_hybrid(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, int max_passes,
                          double resid_max, int use_surrogate);
+/* cable_gpu_run_hybrid that allocates nothing.  No counterpart in the
+   reference: for callers that cannot allocate device memory while they run
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The step's scratch -- the network's predictions and the work
+   and fallback counters -- lives in a device array of the library's code
+   object, sized for at most 1,048,576 tiles (32 MiB, held on each GPU the
+   library runs on until the process ends); a larger context is refused with
+   CABLE_E_ARG.  Everything else must exist before the call: the context and
+   its fields (cable_gpu_create), class tables and forcing, and, with
+   use_surrogate = 1, the network (cable_gpu_set_surrogate, which also sets up
+   the surrogate's state; without it the call returns CABLE_E_STATE).  Every
+   call on a GPU shares that GPU's scratch: calls that overlap on one GPU
+   race.  Results, counters (cable_gpu_hybrid_counters, cable_gpu_counters)
+   and cable_gpu_last_run_ms as cable_gpu_run_hybrid; no per-kernel split
+   (cable_gpu_last_kernel_ms gives the whole run as step time). */
+int cable_gpu_run_hybrid_noalloc(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, int max_passes,
+                                 double resid_max, int use_surrogate);
 /* tile-steps that fell back, and all tile-steps, of the last hybrid run */
 int cable_gpu_hybrid_counters(cable_gpu_ctx *ctx, int64_t *fallback_tile_steps, int64_t *tile_steps);
 /* The surrogate's view of step k from the current state, as the hybrid step
