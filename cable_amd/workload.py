@@ -111,6 +111,42 @@ def tile_classes(n: int, seed: int = DEFAULT_SEED):
     return iveg.astype(np.int32), isoilm.astype(np.int32)
 
 
+def fortran_nint(x):
+    """Fortran NINT: the nearest integer, halves rounded away from zero."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.floor(np.abs(x) + 0.5) * np.sign(x)
+
+
+def froot_from_rootbeta(rootbeta, zse):
+    """Root fraction per soil layer from the PFT's rootbeta (Jackson et al.
+    1996): the cumulative fraction 1 - rootbeta**d at the bottom of layers
+    1..5 (d in cm), the last layer the rest, then differenced
+    (init_veg_from_vegin, src/offline/cable_parameters.F90:3333-3342).
+    Returns (6, n)."""
+    rootbeta = np.atleast_1d(np.asarray(rootbeta, dtype=np.float64))
+    froot = np.zeros((MS, rootbeta.size))
+    totdepth = 0.0
+    for k in range(MS - 1):
+        totdepth = totdepth + zse[k] * 100.0
+        froot[k] = np.minimum(1.0, 1.0 - np.power(rootbeta, totdepth))
+    froot[MS - 1] = 1.0 - froot[MS - 2]
+    for k in range(MS - 2, 0, -1):
+        froot[k] = froot[k] - froot[k - 1]
+    return froot
+
+
+def initial_owetfac(wb1, wbice1, swilt, sfc):
+    """The surface wetness factor's starting value: the top layer's water
+    between wilting point and field capacity, clamped to [0, 1], reduced by
+    (1 - ice fraction)**2 where the layer holds ice (derived_parameters,
+    src/offline/cable_parameters.F90:2245-2254)."""
+    wb1, wbice1 = np.asarray(wb1, dtype=np.float64), np.asarray(wbice1, dtype=np.float64)
+    o = np.maximum(0.0, np.minimum(1.0, (wb1 - swilt) / (sfc - swilt)))
+    ice = wbice1 > 0.0
+    frac = np.where(ice, wbice1 / np.where(ice, wb1, 1.0), 0.0)
+    return np.where(ice, o * (1.0 - frac) ** 2, o)
+
+
 def veg_class_table():
     """Per-PFT vegetation parameters as cbm sees them (after
     init_veg_from_vegin and write_default_params).  Returns {name: array(17)}
@@ -129,16 +165,7 @@ def veg_class_table():
     out["ejmax"] = 2.0 * V["vcmax"]
     out["refl"] = np.stack([V["refl1"], V["refl2"], V["refl3"]])
     out["taul"] = np.stack([V["taul1"], V["taul2"], V["taul3"]])
-    # froot from rootbeta, cable_parameters.F90:3331-3341
-    froot = np.zeros((MS, T.NVEG))
-    totdepth = 0.0
-    for k in range(MS - 1):
-        totdepth = totdepth + T.ZSE[k] * 100.0
-        froot[k] = np.minimum(1.0, 1.0 - np.power(V["rootbeta"], totdepth))
-    froot[MS - 1] = 1.0 - froot[MS - 2]
-    for k in range(MS - 2, 0, -1):
-        froot[k] = froot[k] - froot[k - 1]
-    out["froot"] = froot
+    out["froot"] = froot_from_rootbeta(V["rootbeta"], T.ZSE)
     out["cplant"] = np.stack([V["cplant1"], V["cplant2"], V["cplant3"]])
     out["csoil"] = np.stack([V["csoil1"], V["csoil2"]])
     out["ratecp"] = np.array([V["ratecp1"][0], V["ratecp2"][0], V["ratecp3"][0]])
@@ -159,7 +186,7 @@ def soil_class_table():
     out["css"][ice] = 2100.0        # csice
     # derived_parameters, cable_parameters.F90:2236-2241
     out["hsbh"] = out["hyds"] * np.abs(out["sucs"]) * out["bch"]
-    nint = np.floor(np.abs(out["bch"]) + 0.5) * np.sign(out["bch"])
+    nint = fortran_nint(out["bch"])
     out["ibp2"] = nint + 2.0
     out["pwb_min"] = np.power(out["swilt"] / out["ssat"], out["ibp2"])
     out["i2bp3"] = 2.0 * nint + 3.0
@@ -268,7 +295,7 @@ class Workload:
         s["bgc%cplant"] = self.vtab["cplant"][:, self.iveg - 1]
         s["bgc%csoil"] = self.vtab["csoil"][:, self.iveg - 1]
         # derived_parameters, cable_parameters.F90:2245-2254, 2296-2302
-        s["ssnow%owetfac"] = np.maximum(0.0, np.minimum(1.0, (wb[0] - swilt) / (sfc - swilt)))
+        s["ssnow%owetfac"] = initial_owetfac(wb[0], s["ssnow%wbice"][0], swilt, sfc)
         s["ssnow%wbliq"] = wb - s["ssnow%wbice"]
         s["ssnow%wblf"] = np.maximum(0.01, s["ssnow%wbliq"] / ssat[None, :])
         s["ssnow%wbfice"] = s["ssnow%wbice"] / ssat[None, :]
