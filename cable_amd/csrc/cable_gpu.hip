@@ -357,16 +357,21 @@ int cable_gpu_download(cable_gpu_ctx *c, int id, void *host, int host_type) {
 
 namespace {
 // header of a saved state: magic, layout version, mp, precision, field count
-// and the field table's checksum, then each saved field's (id, n, element
-// size), then the fields' device-typed values in the caller's tile order
+// and the field table's checksum (32 bytes); then per saved field a 16-byte
+// record (id, n, element size, 0) and the field's device-typed values in the
+// caller's tile order, padded to a multiple of 8 bytes, so that every
+// record and every array starts 8-byte aligned
 constexpr uint32_t STATE_MAGIC = 0x534C4243u;  // "CBLS"
-constexpr uint32_t STATE_VERSION = 1;
+constexpr uint32_t STATE_VERSION = 2;
+constexpr int64_t STATE_REC_BYTES = 4 * sizeof(int32_t);
+int64_t pad8(int64_t b) { return (b + 7) & ~(int64_t)7; }
 struct StateHeader {
   uint32_t magic, version;
   int64_t mp;
   int32_t precision, nsaved;
   uint64_t table_sum;
 };
+static_assert(sizeof(StateHeader) == 32, "the saved state's header is 32 bytes");
 bool saved_field(int id) { return kFields[id].role == ROLE_STATE || kFields[id].role == ROLE_DIAG; }
 uint64_t table_checksum() {
   uint64_t h = 1469598103934665603ull;  // FNV-1a over names, precisions and counts
@@ -380,7 +385,7 @@ uint64_t table_checksum() {
 int64_t state_bytes(const cable_gpu_ctx *c) {
   int64_t b = sizeof(StateHeader);
   for (int id = 0; id < CABLE_NFIELDS; id++)
-    if (saved_field(id)) b += 3 * sizeof(int32_t) + (int64_t)field_bytes(c, id);
+    if (saved_field(id)) b += STATE_REC_BYTES + pad8((int64_t)field_bytes(c, id));
   return b;
 }
 int host_type_of(const FieldMeta &f, int precision) {
@@ -409,12 +414,14 @@ int cable_gpu_save_state(cable_gpu_ctx *c, void *buf, int64_t bytes) {
   for (int id = 0; id < CABLE_NFIELDS; id++) {
     if (!saved_field(id)) continue;
     const FieldMeta &f = kFields[id];
-    const int32_t rec[3] = {id, f.n, (int32_t)dev_elem(f, c->precision)};
+    const int32_t rec[4] = {id, f.n, (int32_t)dev_elem(f, c->precision), 0};
     memcpy(p, rec, sizeof rec);
     p += sizeof rec;
     int r = cable_gpu_download(c, id, p, host_type_of(f, c->precision));
     if (r) return r;
-    p += field_bytes(c, id);
+    const int64_t fb = (int64_t)field_bytes(c, id);
+    memset(p + fb, 0, (size_t)(pad8(fb) - fb));
+    p += pad8(fb);
   }
   return CABLE_OK;
 }
@@ -435,7 +442,7 @@ int cable_gpu_load_state(cable_gpu_ctx *c, const void *buf, int64_t bytes, int c
   if (bytes < state_bytes(c)) return fail(CABLE_E_ARG, "cable_gpu_load_state: truncated state");
   p += sizeof h;
   for (int i = 0; i < h.nsaved; i++) {
-    int32_t rec[3];
+    int32_t rec[4];
     memcpy(rec, p, sizeof rec);
     p += sizeof rec;
     const int id = rec[0];
@@ -444,7 +451,7 @@ int cable_gpu_load_state(cable_gpu_ctx *c, const void *buf, int64_t bytes, int c
       return fail(CABLE_E_ARG, "cable_gpu_load_state: bad field record %d", i);
     int r = cable_gpu_upload(c, id, p, host_type_of(kFields[id], c->precision));
     if (r) return r;
-    p += field_bytes(c, id);
+    p += pad8((int64_t)field_bytes(c, id));
   }
   // an exact continuation keeps the run's first-call state; otherwise the next
   // step is the first of a new run, as a reference run started from a restart
