@@ -23,6 +23,14 @@
 ! some carried values exactly (e.g. ssnow%cls against 1.1335 in
 ! src/science/soilsnow/cbl_snowAccum.F90), and those tests flip when an FP64
 ! value is rounded to FP32 between steps.
+!
+! The device implements the shipped configuration (src/offline/cable.nml):
+! icycle = 0 and the cable_user switches the path's science branches on at
+! their shipped values (soil_struc 'default', fwsoil_switch 'standard',
+! gs_switch 'leuning', ssnow_potev 'HDM', diag_soil_resp 'ON', every logical
+! option off, snmin = 1, no hydraulic redistribution).  The first call checks
+! them before it touches the device and stops with a message (ERROR STOP 999,
+! as cable_abort does) if the run asks for anything else.
 MODULE cable_cbm_module
 USE, INTRINSIC :: ISO_C_BINDING
 USE cable_gpu_c
@@ -62,6 +70,7 @@ SUBROUTINE cbm(ktau, dels, air, bgc, canopy, met, bal, rad, rough, soil, &
   REAL(C_DOUBLE) :: zse(ms), zshh(ms+1), ratecp(ncp), ratecs(ncs)
 
   IF (.NOT. C_ASSOCIATED(ctx)) THEN
+    CALL check_configuration()
     IF (cable_gpu_precision < 0) &
       cable_gpu_precision = MERGE(CABLE_PREC_FP64, CABLE_PREC_MIXED, KIND(1.0) == 8)
     CALL cable_gpu_check(cable_gpu_create(ctx, INT(mp, C_INT64_T), cable_gpu_precision))
@@ -75,6 +84,41 @@ SUBROUTINE cbm(ktau, dels, air, bgc, canopy, met, bal, rad, rough, soil, &
   CALL cable_gpu_check(cable_gpu_cbm(ctx, INT(ktau, C_INT), REAL(dels, C_DOUBLE)))
   CALL cable_gpu_get_fields(ctx, air, bgc, canopy, met, bal, rad, rough, soil, ssnow, veg)
 END SUBROUTINE cbm
+
+! The run's configuration against the one the device implements; stops on
+! the first difference (before any device call).
+SUBROUTINE check_configuration()
+  USE cable_runtime_opts_mod, ONLY: cable_user, snmin
+  USE cable_common_module, ONLY: redistrb
+  USE casadimension, ONLY: icycle
+  CALL need(icycle == 0, 'icycle = 0 (no CASA-CNP)')
+  CALL need(TRIM(cable_user%soil_struc) == 'default', "cable_user%soil_struc = 'default'")
+  CALL need(TRIM(cable_user%fwsoil_switch) == 'standard', "cable_user%fwsoil_switch = 'standard'")
+  CALL need(TRIM(cable_user%gs_switch) == 'leuning', "cable_user%gs_switch = 'leuning'")
+  CALL need(TRIM(cable_user%ssnow_potev) == 'HDM', "cable_user%ssnow_potev = 'HDM'")
+  CALL need(TRIM(cable_user%diag_soil_resp) == 'ON', "cable_user%diag_soil_resp = 'ON'")
+  CALL need(.NOT. cable_user%l_rev_corr, 'cable_user%l_rev_corr off')
+  CALL need(.NOT. cable_user%l_new_roughness_soil, 'cable_user%l_new_roughness_soil off')
+  CALL need(.NOT. cable_user%l_new_runoff_speed, 'cable_user%l_new_runoff_speed off')
+  CALL need(.NOT. cable_user%l_new_reduce_soilevp, 'cable_user%l_new_reduce_soilevp off')
+  CALL need(.NOT. cable_user%soil_thermal_fix, 'cable_user%soil_thermal_fix off')
+  CALL need(.NOT. cable_user%or_evap, 'cable_user%or_evap off')
+  CALL need(.NOT. cable_user%gw_model, 'cable_user%gw_model off')
+  CALL need(.NOT. cable_user%litter, 'cable_user%litter off')
+  CALL need(.NOT. cable_user%call_climate, 'cable_user%call_climate off')
+  CALL need(.NOT. cable_user%soilsnow_init_spec, 'cable_user%soilsnow_init_spec off')
+  CALL need(.NOT. redistrb, 'redistrb off (no hydraulic redistribution)')
+  CALL need(snmin == 1.0, 'snmin = 1')
+END SUBROUTINE check_configuration
+
+SUBROUTINE need(ok, what)
+  LOGICAL, INTENT(IN) :: ok
+  CHARACTER(LEN=*), INTENT(IN) :: what
+  IF (.NOT. ok) THEN
+    WRITE(*, '(A)') 'cbm (GPU): the run asks for a configuration the device does not implement; it needs ' // what
+    ERROR STOP 999
+  END IF
+END SUBROUTINE need
 
 ! release the device copy (optional; the process end releases it too)
 SUBROUTINE cbm_gpu_finalise()

@@ -17,6 +17,10 @@
 !           op 3 STEP  : int32 ktau, real(8) dels
 !           op 4 GET   : char(40) name   -> output record: char(40) name, int64 n, data
 !           op 5 NAMES : write the component list to stdout
+!           op 7 SWITCH: char(40) switch, char(40) value -- a configuration
+!                        switch away from the shipped one (a cable_user
+!                        character switch, a cable_user logical 'T'/'F',
+!                        or icycle)
 !           op 0 END
 PROGRAM cable_ref_driver
 USE cable_def_types_mod
@@ -35,7 +39,7 @@ INTEGER(8) :: n, mp8
 INTEGER(4) :: ktau4
 REAL(8) :: dels8
 REAL :: dels
-CHARACTER(LEN=40) :: name
+CHARACTER(LEN=40) :: name, value
 REAL(8), ALLOCATABLE :: buf(:)
 TYPE (climate_type) :: climate
 TYPE (casa_flux) :: casaflux   ! not referenced with icycle = 0
@@ -143,6 +147,9 @@ DO
     WRITE(uout) name, n, buf
   CASE (5)
     CALL field_names(6)
+  CASE (7)
+    READ(uin) name, value
+    CALL set_switch(name, value)
   CASE (6)
     ! wall time spent in the step operations (cbm and the driver contract
     ! around it), excluding file input and output
@@ -157,6 +164,26 @@ CLOSE(uin)
 CLOSE(uout)
 
 CONTAINS
+
+SUBROUTINE set_switch(sw, val)
+  CHARACTER(LEN=*), INTENT(IN) :: sw, val
+  LOGICAL :: on
+  on = TRIM(val) == 'T'
+  SELECT CASE (TRIM(sw))
+  CASE ('gs_switch'); cable_user%gs_switch = TRIM(val)
+  CASE ('fwsoil_switch'); cable_user%fwsoil_switch = TRIM(val)
+  CASE ('ssnow_potev'); cable_user%ssnow_potev = TRIM(val)
+  CASE ('soil_struc'); cable_user%soil_struc = TRIM(val)
+  CASE ('diag_soil_resp'); cable_user%diag_soil_resp = TRIM(val)
+  CASE ('or_evap'); cable_user%or_evap = on
+  CASE ('litter'); cable_user%litter = on
+  CASE ('gw_model'); cable_user%gw_model = on
+  CASE ('icycle'); READ(val, *) icycle
+  CASE DEFAULT
+    WRITE(0,*) 'unknown switch ', TRIM(sw)
+    ERROR STOP 5
+  END SELECT
+END SUBROUTINE set_switch
 
 ! The balance accumulators of src/offline/cable_checks.F90 cannot be compiled
 ! here (the module USEs cable_abort_module, which needs netCDF), so the driver

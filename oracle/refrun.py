@@ -46,7 +46,7 @@ def _set(f, name, arr):
 
 
 def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
-        extra_sets=None, timeout=3600, poison=None, cpu=None):
+        extra_sets=None, timeout=3600, poison=None, cpu=None, switches=None):
     """Run ``nsteps`` steps of the reference on workload ``wl``.
 
     outputs: field names to read back; with every_step they are read after
@@ -54,6 +54,8 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
     poison: optional (step, name, value) -- overwrite a field before that
     step (0-based), used to find which fields carry state between steps.
     cpu: optional host core to pin the driver process to (taskset).
+    switches: optional {switch: value} away from the shipped configuration
+    (the driver's op 7), set before the first step.
     Returns (dict, seconds the driver spent in its step operations -- cbm
     and the driver contract around it -- measured inside the driver process,
     without its file input and output).
@@ -70,6 +72,8 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
             for k, v in (extra_sets or {}).items():
                 _set(f, k, v)
             f.write(struct.pack("<i", 2))
+            for k, v in (switches or {}).items():
+                f.write(struct.pack("<i", 7) + _name(k) + _name(str(v)))
             fr = wl.forcing if wl.forcing is not None else wl.make_forcing(0, nsteps)
             for s in range(nsteps):
                 for k, v in fr.items():
@@ -87,8 +91,8 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
         # arrays of mp elements), so a large mp needs the stack limit raised,
         # as the offline model is run with `ulimit -s unlimited`
         pin_cmd = ["taskset", "-c", str(cpu)] if cpu is not None else []
-        subprocess.run(pin_cmd + [driver_path(build), pin, pout], check=True,
-                       timeout=timeout, stdout=subprocess.DEVNULL, preexec_fn=_big_stack)
+        subprocess.run(pin_cmd + [driver_path(build), pin, pout], check=True, timeout=timeout,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, preexec_fn=_big_stack)
         dt = time.perf_counter() - t0
         res = {}
         with open(pout, "rb") as f:

@@ -126,3 +126,19 @@ def test_shipped_precision_driver_through_fortran_dropin():
         print(f"{k}: |gpu4 - r8| {d_gpu:.3e}  |gpu4 - r4| {d_gr4:.3e}  |r4 - r8| {d_r4:.3e}", flush=True)
         assert d_gpu <= 1.0, (k, d_gpu)
         assert d_gr4 <= max(2.0 * d_r4, 0.05), (k, d_gr4, d_r4)
+
+
+@pytest.mark.skipif(not refrun.available("gpu8"), reason="oracle/_ref/gpu8 not built")
+@pytest.mark.parametrize("switch,value", [("gs_switch", "medlyn"), ("icycle", "1"), ("or_evap", "T"),
+                                          ("soil_struc", "sli")])
+def test_dropin_refuses_a_configuration_it_does_not_implement(switch, value):
+    """The reference driver built with the GPU drop-in, run with one switch
+    away from the shipped configuration: the first cbm call stops (ERROR STOP
+    999) naming the switch, before any device call -- so this runs without a
+    GPU."""
+    import subprocess
+    case, _ = golden()
+    with pytest.raises(subprocess.CalledProcessError) as e:
+        refrun.run(case, 1, [], every_step=False, build="gpu8", switches={switch: value}, timeout=120)
+    out = (e.value.stdout or b"").decode() + (e.value.stderr or b"").decode()
+    assert "does not implement" in out and switch.split("_")[0] in out, out[-500:]
