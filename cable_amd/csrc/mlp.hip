@@ -1,17 +1,44 @@
 // mlp.hip -- the canopy surrogate: a 24 -> 128 -> 128 -> 128 -> 8 MLP with
-// SiLU, BF16 operands on the matrix cores (MFMA), FP32 accumulation, FP32
-// master weights and Adam.  All kernels are hand-written for gfx950 wave64.
+// SiLU, trained on the GPU: BF16 operands on the matrix cores
+// (v_mfma_f32_32x32x16_bf16) with FP32 accumulation, FP32 master weights and
+// Adam.  All kernels are hand-written for gfx950 wave64.
 //
-// Matrix products use v_mfma_f32_32x32x8_bf16: one wave computes a 32x32
-// block of C = A.B, stepping K by 8.  Lane l holds
-//   A[i][k] for i = l % 32, k = 4*(l/32) + 0..3
-//   B[k][j] for j = l % 32, k = 4*(l/32) + 0..3
-//   C[i][j] for j = l % 32, i = (r % 4) + 4*(l/32) + 8*(r/4), r = 0..15.
-// Operands are read through strides, so the same kernel forms X.W, dZ.W^T
-// and H^T.dZ; every element outside [0,M) x [0,K) x [0,N) reads as zero.
+// A training batch takes two launches.
+//
+// rows_kernel: a workgroup of four waves takes the batch 32 rows at a time.
+// It gathers and normalises their samples, runs the forward pass and the
+// backward pass down to the first layer's pre-activation gradient, and forms
+// its rows' share of every parameter's gradient.  Wave w owns features
+// 32w .. 32w+31 of every 128-wide layer in both passes, so the SiLU
+// derivative it needs on the way back is the one it computed on the way
+// forward, still in its registers; activations and gradients pass between
+// layers through LDS (one workgroup barrier per layer).  Each wave loads its
+// slices of the BF16 weights into registers once per launch.  The layer
+// inputs H and the pre-activation gradients dZ of the 32 rows are also kept
+// in LDS feature-major, and the workgroup forms dW = H^T dZ over its rows on
+// the matrix cores and the bias gradients as FP32 sums of dZ, into its own
+// slot of a partial-gradient buffer (a workgroup that takes several blocks
+// of rows adds them up there).
+//
+// update_kernel: four threads per parameter sum the workgroups' partials in
+// a fixed order and take the Adam step (or store the gradient), then
+// refresh the BF16 copies the next batch loads.  Nothing is summed by
+// atomics: a batch's gradient, loss and update are the same on every run.
+//
+// At 2,048 rows a batch runs 64 workgroups and is bound by latency, not by
+// the matrix cores: the rows pass is a chain of seven dependent products,
+// their epilogues and barriers, and the update waits on loads of partials
+// written by every XCD.
+//
+// A product is computed transposed, D = W^T X^T (W [in][out], X [row][in]):
+// lane l supplies A[l%32][k0 + 8(l/32) .. +7] and B[k0 + 8(l/32) .. +7][l%32],
+// eight consecutive BF16 each, and receives D[8g + 4(l/32) + q][l%32] in
+// register 4g + q, so a lane holds four consecutive features of one row (row
+// l%32) per register group.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -45,136 +72,487 @@ int fail(int code, const char *fmt, ...) {
 // ---------------------------------------------------------------------------
 // kernels
 
-enum Epi { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_SILU_GRAD = 3 };
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 
-// C[M x N] = A[M x K] . B[K x N]; one wave per 32x32 block of C.  With
-// kchunk < K the product is split along K: block z sums k in
-// [z*kchunk, (z+1)*kchunk) and stores its partial C at C + z*M*N (EPI_STORE
-// only); reduce_kernel adds the partials in order.
-template <class TA, class TB>
-__global__ void __launch_bounds__(64) gemm_kernel(const TA *__restrict__ A, int64_t sam, int64_t sak,
-                                                  const TB *__restrict__ B, int64_t sbk, int64_t sbn,
-                                                  float *__restrict__ C, float *__restrict__ H, int M,
-                                                  int N, int K, const float *__restrict__ bias,
-                                                  const float *__restrict__ Zsaved, int epi, int kchunk) {
-  const int lane = threadIdx.x;
-  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
-  const int ia = i0 + (lane & 31), jb = j0 + (lane & 31), kq = 4 * (lane >> 5);
-  const int kbeg = blockIdx.z * kchunk, kend = kbeg + kchunk < K ? kbeg + kchunk : K;
-  C += (int64_t)blockIdx.z * M * N;
-  v16f acc;
+constexpr int RW = 32;                 // rows per block
+constexpr int NWAVE = 4;               // waves of rows_kernel, one per 32 features
+constexpr int AP = HID + 8;            // row-major LDS pitch (BF16): 68 dwords, conflict-free 16-byte reads
+constexpr int TP = RW + 8;             // feature-major LDS pitch (BF16): 20 dwords, likewise
+constexpr int K1 = 32;                 // layer 1's K (24 inputs) padded to a multiple of 16
+constexpr int N4B = 16;                // layer 4's 8 outputs padded, as the K of the backward product
+// BF16 working copies, zero-padded.  wT: W^T [out][in] of every layer for
+// the forward pass -- W1^T [128][32], W2^T and W3^T [128][128], W4^T
+// [32][128] with rows 8..31 zero.  wB: W [in][out] of layers 4, 3, 2 for the
+// backward pass -- W4 [128][16] with columns 8..15 zero, W3 and W2 [128][128].
+constexpr int WT_OFF[4] = {0, HID * K1, HID * K1 + HID * HID, HID * K1 + 2 * HID * HID};
+constexpr int WT_SIZE = WT_OFF[3] + 32 * HID;
+constexpr int WB_OFF[4] = {-1, HID * N4B + HID * HID, HID * N4B, 0};
+constexpr int WB_SIZE = HID * N4B + 2 * HID * HID;
+// feature-major LDS rows of a block: the inputs of layers 1..4 (layer 1's
+// padded to 32) and the pre-activation gradients of layers 1..4 (layer 4's
+// padded to 32); the pads are zero
+constexpr int HT_OFF[4] = {0, K1, K1 + HID, K1 + 2 * HID};
+constexpr int HT_ROWS = K1 + 3 * HID;
+constexpr int DT_OFF[4] = {0, HID, 2 * HID, 3 * HID};
+constexpr int DT_ROWS = 3 * HID + 32;
+// 32x32 blocks of the weight gradients: layer 1 (one input block, 24 of its
+// 32 rows real) x 4, layers 2 and 3 4 x 4, layer 4 4 x (one output block, 8
+// of its 32 columns real)
+constexpr int NBLOCKS = 4 + 16 + 16 + 4;
+constexpr int NBIAS = 3 * HID + NOUT;
+// a workgroup's slot of partial gradients, in the order the matrix cores
+// leave them: the 40 weight-gradient blocks, each as the 16 registers of its
+// 64 lanes (so a register's store is 256 contiguous bytes and a block's 4 KB;
+// the pad rows and columns are stored too and never read), then the biases
+// in layer order, then the sum of squared errors; padded to 256 bytes.
+// (Stored in the parameter vector's own layout, each register's lanes write
+// two 128-byte pieces 512 bytes apart and the stores took twice as long as
+// the rest of the pass.)
+constexpr int PBLK = 16 * 64;
+constexpr int PB_OFF = NBLOCKS * PBLK;
+constexpr int PLOSS = PB_OFF + NBIAS;
+constexpr int PSTRIDE = (PLOSS + 1 + 63) / 64 * 64;
+constexpr int GMAX = 256;              // most workgroups of rows_kernel: one per CU
+
+struct RowsArgs {
+  const float *xs, *ys;      // raw samples gathered through idx (training), or
+  const int64_t *idx;        // through the shuffled order (shuffled) ...
+  int64_t nsamples, pos0;    // ... positions pos0 + row of it
+  uint64_t seed;
+  int shuffled;
+  const float *x, *y;        // ... or normalised rows (idx null)
+  const float *norm;
+  const float *w;            // FP32 parameters (the biases are read here)
+  const uint16_t *wT, *wB;   // BF16 copies
+  float *out;                // forward only: outputs [n][8]
+  float *part;               // with the backward pass: partial gradients [gridDim.x][PSTRIDE]
+  int n, backward;
+};
+
+struct UpdateArgs {
+  const float *part;
+  int npart, n;
+  float *w, *m, *v, *g;      // parameters, Adam's moments, gradient
+  uint16_t *wT, *wB;
+  float *loss;               // the batch's mean-squared error (optional)
+  int adam;                  // 1: Adam step at rate lr; 0: store the gradient in g
+  float lr, b1, b2, eps, c1, c2;
+};
+
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+// the sum, over the 32 lanes of a half-wave, of each of a lane's NV values.
+// Each exchange step halves the values a lane carries: the lane keeps one
+// half, adds its partner's copy of it and passes on the other half; after
+// log2(NV) steps the rest is a plain butterfly.  The lane returns the total
+// of value number vi; lanes that differ only in their low 5 - log2(NV) bits
+// return the same total.  NV - 1 + 5 - log2(NV) exchanges in all.
+template <int NV>
+struct floats {
+  typedef float type __attribute__((ext_vector_type(NV)));
+};
+
+// (the empty asm makes the two values opaque: without it the compiler folds
+// the selects into a run-time index into v, a chain of compares over every
+// element)
+template <int NV>
+__device__ __forceinline__ float half_sums(typename floats<NV>::type v, int lane, int &vi) {
+  vi = 0;
+  int m = 16;
 #pragma unroll
-  for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-  for (int k0 = kbeg; k0 < kend; k0 += 8) {
-    v4s a, b;
+  for (int h = NV / 2; h >= 1; h /= 2, m >>= 1) {
+    const bool up = (lane & m) != 0;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int k = k0 + kq + e;
-      const float av = (ia < M && k < kend) ? to_f32(A[(int64_t)ia * sam + (int64_t)k * sak]) : 0.0f;
-      const float bv = (jb < N && k < kend) ? to_f32(B[(int64_t)k * sbk + (int64_t)jb * sbn]) : 0.0f;
-      a[e] = (short)f32_to_bf16(av);
-      b[e] = (short)f32_to_bf16(bv);
+    for (int t = 0; t < h; t++) {
+      float lo = v[t], hi = v[t + h];
+      asm("" : "+v"(lo), "+v"(hi));
+      const float send = up ? lo : hi;
+      const float keep = up ? hi : lo;
+      v[t] = keep + __shfl_xor(send, m, 64);
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    vi += up ? h : 0;
   }
-  const int j = j0 + (lane & 31);
-  if (j >= N) return;
+  float s = v[0];
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const int i = i0 + (r & 3) + 4 * (lane >> 5) + 8 * (r >> 2);
-    if (i >= M) continue;
-    const int64_t o = (int64_t)i * N + j;
-    float v = acc[r];
-    switch (epi) {
-      case EPI_BIAS_SILU: {
-        v += bias[j];
-        C[o] = v;             // pre-activation, kept for the backward pass
-        H[o] = silu(v);
-        break;
+  for (; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  return s;
+}
+
+// the block's contribution to partial gradient k of this workgroup (the
+// first block it takes stores, later ones add; always the same thread)
+__device__ __forceinline__ void part_add(const RowsArgs &p, int k, float v, bool first) {
+  float *q = p.part + (int64_t)blockIdx.x * PSTRIDE + k;
+  *q = first ? v : *q + v;
+}
+
+__device__ __forceinline__ v16f zero16() {
+  v16f a;
+#pragma unroll
+  for (int r = 0; r < 16; r++) a[r] = 0.0f;
+  return a;
+}
+
+// D = W^T X^T for this wave's 32 output features over the block's 32 rows;
+// KS steps of K 16.  Two accumulators halve the dependent chain.
+template <int KS>
+__device__ __forceinline__ v16f product(const bf8 *W, const __bf16 *in, int col, int kh) {
+  v16f a0 = zero16(), a1 = zero16();
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    const bf8 xv = *(const bf8 *)&in[col * AP + 16 * s + kh];
+    if (s & 1)
+      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[s], xv, a1, 0, 0, 0);
+    else
+      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[s], xv, a0, 0, 0, 0);
+  }
+  if (KS > 1) a0 += a1;
+  return a0;
+}
+
+// a hidden layer forward: z = XW + b, h = silu(z) into `out` (and into the
+// block's feature-major H rows `ht` when the backward pass follows), with
+// silu'(z) kept in sg
+template <int KS>
+__device__ __forceinline__ void forward_layer(const bf8 *W, const __bf16 *in, __bf16 *out, const float *b,
+                                              float *sg, __bf16 *ht, int wave, int col, int kh, int kq) {
+  const v16f acc = product<KS>(W, in, col, kh);
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const int j = 32 * wave + 8 * g + kq;
+    bf4 hv;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float z = acc[4 * g + q] + b[j + q];
+      const float s = sigmoid(z);
+      sg[4 * g + q] = s * (1.0f + z * (1.0f - s));
+      hv[q] = (__bf16)(z * s);
+    }
+    *(bf4 *)&out[col * AP + j] = hv;
+    if (ht)
+#pragma unroll
+      for (int q = 0; q < 4; q++) ht[(j + q) * TP + col] = hv[q];
+  }
+}
+
+// a hidden layer backward: dZ = (dZ_next W^T) silu'(z), into `out` (when a
+// product follows) and the block's feature-major dZ rows `dt`; the bias
+// gradient's FP32 share from this block into the partials at bias0
+template <int KS>
+__device__ __forceinline__ void backward_layer(const RowsArgs &p, const bf8 *W, const __bf16 *in, __bf16 *out,
+                                               const float *sg, __bf16 *dt, int bias0, bool first, int wave,
+                                               int lane, int col, int kh, int kq) {
+  const v16f acc = product<KS>(W, in, col, kh);
+  v16f dz;
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const int i = 32 * wave + 8 * g + kq;
+    bf4 dv;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      dz[4 * g + q] = acc[4 * g + q] * sg[4 * g + q];
+      dv[q] = (__bf16)dz[4 * g + q];
+    }
+    if (out) *(bf4 *)&out[col * AP + i] = dv;
+#pragma unroll
+    for (int q = 0; q < 4; q++) dt[(i + q) * TP + col] = dv[q];
+  }
+  int vi;
+  const float s = half_sums<16>(dz, lane, vi);
+  if ((lane & 1) == 0) part_add(p, bias0 + 32 * wave + 8 * (vi >> 2) + kq + (vi & 3), s, first);
+}
+
+struct RowsLDS {
+  __attribute__((aligned(16))) __bf16 act[2][RW * AP];
+  __attribute__((aligned(16))) __bf16 Ht[HT_ROWS * TP];
+  __attribute__((aligned(16))) __bf16 Dt[DT_ROWS * TP];
+  float bias[NBIAS];
+  float tgt[RW * NOUT];
+};
+
+// this wave's share of layer L's weight gradient over the block's rows,
+// dW = H^T dZ with K = the 32 rows, into the workgroup's slot of partials:
+// the 32x32 blocks (ib, jb) with ib = wave (layers 2-4: input block) or
+// jb = wave (layer 1: output block).  A later block's loads all go before
+// its stores.  The stores are non-temporal: the next kernel reads them on
+// every XCD, so nothing gains from keeping them in this XCD's L2 (measured
+// 3 us less per pass at 64 workgroups).
+template <int L>
+__device__ __forceinline__ void weight_grad_blocks(const RowsArgs &p, const __bf16 *Ht, const __bf16 *Dt, int wave,
+                                                   bool first, int lane, int col, int kh) {
+  constexpr int NJ = L == 0 ? 1 : L == 3 ? 1 : 4;      // blocks per wave
+  constexpr int B0 = L == 0 ? 0 : L == 1 ? 4 : L == 2 ? 20 : 36;
+#pragma unroll
+  for (int t = 0; t < NJ; t++) {
+    const int ib = L == 0 ? 0 : wave, jb = L == 0 ? wave : L == 3 ? 0 : t;
+    const int bk = B0 + (L == 0 ? wave : L == 3 ? wave : 4 * wave + t);
+    v16f acc = zero16();
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const bf8 av = *(const bf8 *)&Ht[(HT_OFF[L] + 32 * ib + col) * TP + 16 * s + kh];
+      const bf8 bv = *(const bf8 *)&Dt[(DT_OFF[L] + 32 * jb + col) * TP + 16 * s + kh];
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    float *dst = p.part + (int64_t)blockIdx.x * PSTRIDE + bk * PBLK + lane;
+    v16f old = zero16();
+    if (!first)
+#pragma unroll
+      for (int r = 0; r < 16; r++) old[r] = dst[64 * r];
+#pragma unroll
+    for (int r = 0; r < 16; r++) __builtin_nontemporal_store(first ? acc[r] : old[r] + acc[r], &dst[64 * r]);
+  }
+}
+
+// a workgroup barrier that orders LDS only: __syncthreads() would also wait
+// for every global load and store in flight (the weights still arriving,
+// the partial gradients still being written)
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// row `row`'s normalised inputs 3c8 .. 3c8+2 and target c8 (zero past n)
+struct Inputs {
+  float x[3], t;
+};
+__device__ __forceinline__ Inputs load_inputs(const RowsArgs &p, int row, int c8) {
+  Inputs in = {{0.0f, 0.0f, 0.0f}, 0.0f};
+  if (row < p.n) {
+    if (p.idx || p.shuffled) {
+      const int64_t s = p.shuffled ? shuffled_sample(p.seed, p.pos0 + row, p.nsamples) : p.idx[row];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const int k = 3 * c8 + c;
+        in.x[c] = (p.xs[s * NIN + k] - p.norm[k]) / p.norm[NIN + k];
       }
-      case EPI_BIAS: C[o] = v + bias[j]; break;
-      case EPI_SILU_GRAD: C[o] = v * silu_grad(Zsaved[o]); break;
-      default: C[o] = v; break;
+      if (p.backward) in.t = (p.ys[s * NOUT + c8] - p.norm[2 * NIN + c8]) / p.norm[2 * NIN + NOUT + c8];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) in.x[c] = p.x[(int64_t)row * NIN + 3 * c8 + c];
+      if (p.backward) in.t = p.y[(int64_t)row * NOUT + c8];
     }
   }
+  return in;
 }
 
-// out[i] = sum_z part[z][i] for i < n, z < nz (split-K partials, in order)
-__global__ void reduce_kernel(const float *__restrict__ part, int nz, int64_t n, float *__restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float s = 0.0f;
-  for (int z = 0; z < nz; z++) s += part[z * n + i];
-  out[i] = s;
-}
-
-// column sums of dZ in row blocks: block (x, y) sums rows [y*rows, (y+1)*rows)
-// of 32 columns into part[y][j] (eight row groups combined through LDS);
-// reduce_kernel then adds the row blocks in order
-__global__ void __launch_bounds__(256) colsum_kernel(const float *__restrict__ dZ, int M, int N, int rows,
-                                                     float *__restrict__ part) {
-  __shared__ float acc[8][32];
-  const int c = threadIdx.x & 31, r = threadIdx.x >> 5, j = blockIdx.x * 32 + c;
-  const int i0 = blockIdx.y * rows, i1 = i0 + rows < M ? i0 + rows : M;
-  float s = 0.0f;
-  if (j < N)
-    for (int i = i0 + r; i < i1; i += 8) s += dZ[(int64_t)i * N + j];
-  acc[r][c] = s;
-  __syncthreads();
-  if (r == 0 && j < N) {
-    float t = 0.0f;
-    for (int q = 0; q < 8; q++) t += acc[q][c];
-    part[(int64_t)blockIdx.y * N + j] = t;
+// the rows pass of workgroup blockIdx.x of gridDim.x (see the top of the file)
+__device__ __forceinline__ void rows_pass(const RowsArgs &p, RowsLDS &sh) {
+  __bf16(*act)[RW * AP] = sh.act;
+  __bf16 *Ht = sh.Ht, *Dt = sh.Dt;
+  float *bias = sh.bias, *tgt = sh.tgt;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, col = lane & 31;
+  const int kh = 8 * (lane >> 5), kq = 4 * (lane >> 5);
+  const int f = 32 * wave + col;       // the weight row this lane supplies
+  // Loads are issued in the order they are needed: the biases and the first
+  // block's inputs, then the weights layer by layer; the barriers wait for
+  // LDS only, so each layer waits for its own weights and no store waits.
+  float bv[2];
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int e = tid + 64 * NWAVE * r, l = e < 3 * HID ? e / HID : 3;
+    bv[r] = e < NBIAS ? p.w[(l == 0 ? LAYER_B[0] : l == 1 ? LAYER_B[1] : l == 2 ? LAYER_B[2] : LAYER_B[3]) + e -
+                            l * HID]
+                      : 0.0f;
+  }
+  const int nblk = (p.n + RW - 1) / RW;
+  Inputs in = load_inputs(p, blockIdx.x * RW + (tid >> 3), tid & 7);
+  // this wave's weight slices, for the whole launch
+  bf8 w1[2], w2[8], w3[8], w4[8], v4[1], v3[8], v2[8];
+#pragma unroll
+  for (int s = 0; s < 2; s++) w1[s] = *(const bf8 *)&p.wT[WT_OFF[0] + f * K1 + 16 * s + kh];
+#pragma unroll
+  for (int s = 0; s < 8; s++) w2[s] = *(const bf8 *)&p.wT[WT_OFF[1] + f * HID + 16 * s + kh];
+#pragma unroll
+  for (int s = 0; s < 8; s++) w3[s] = *(const bf8 *)&p.wT[WT_OFF[2] + f * HID + 16 * s + kh];
+  // the output layer is wave 0's alone (W4^T rows 8..31 are zero)
+#pragma unroll
+  for (int s = 0; s < 8; s++) w4[s] = *(const bf8 *)&p.wT[WT_OFF[3] + col * HID + 16 * s + kh];
+  if (p.backward) {
+    v4[0] = *(const bf8 *)&p.wB[WB_OFF[3] + f * N4B + kh];
+#pragma unroll
+    for (int s = 0; s < 8; s++) v3[s] = *(const bf8 *)&p.wB[WB_OFF[2] + f * HID + 16 * s + kh];
+#pragma unroll
+    for (int s = 0; s < 8; s++) v2[s] = *(const bf8 *)&p.wB[WB_OFF[1] + f * HID + 16 * s + kh];
+  }
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+    if (tid + 64 * NWAVE * r < NBIAS) bias[tid + 64 * NWAVE * r] = bv[r];
+  // the pads of layer 1's inputs and layer 4's gradients, never written below
+  for (int e = tid; e < (K1 - NIN) * TP; e += 64 * NWAVE) Ht[(HT_OFF[0] + NIN) * TP + e] = (__bf16)0.0f;
+  for (int e = tid; e < (32 - NOUT) * TP; e += 64 * NWAVE) Dt[(DT_OFF[3] + NOUT) * TP + e] = (__bf16)0.0f;
+  const float sc = 2.0f / (float)(p.n * NOUT);
+  float sq = 0.0f;                     // wave 0: squared errors of this workgroup's rows
+  bool first = true;
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x, first = false) {
+    const int r0 = blk * RW;
+    if (!first) {
+      // the last block's readers are done, and (a full barrier) its partial
+      // gradients are stored before this block reads them back
+      __syncthreads();
+      in = load_inputs(p, r0 + (tid >> 3), tid & 7);
+    }
+    // ---- inputs: thread t takes row t/8 of the block, inputs 3(t%8) .. +2
+    // and target t%8
+    {
+      const int rr = tid >> 3, c8 = tid & 7;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const __bf16 b = (__bf16)in.x[c];
+        act[0][rr * AP + 3 * c8 + c] = b;
+        Ht[(HT_OFF[0] + 3 * c8 + c) * TP + rr] = b;
+      }
+      act[0][rr * AP + NIN + c8] = (__bf16)0.0f;
+      tgt[rr * NOUT + c8] = in.t;
+    }
+    lds_barrier();
+    // ---- forward
+    __bf16 *H1 = p.backward ? &Ht[HT_OFF[1] * TP] : nullptr;
+    __bf16 *H2 = p.backward ? &Ht[HT_OFF[2] * TP] : nullptr;
+    __bf16 *H3 = p.backward ? &Ht[HT_OFF[3] * TP] : nullptr;
+    float sg1[16], sg2[16], sg3[16];
+    forward_layer<2>(w1, act[0], act[1], &bias[0], sg1, H1, wave, col, kh, kq);
+    lds_barrier();
+    forward_layer<8>(w2, act[1], act[0], &bias[HID], sg2, H2, wave, col, kh, kq);
+    lds_barrier();
+    forward_layer<8>(w3, act[0], act[1], &bias[2 * HID], sg3, H3, wave, col, kh, kq);
+    lds_barrier();
+    if (wave == 0) {
+      // the lane's register group 0 holds outputs kq .. kq+3 of row col
+      const v16f acc = product<8>(w4, act[1], col, kh);
+      const int row = r0 + col;
+      const bool live = row < p.n;
+      float y[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) y[q] = acc[q] + bias[3 * HID + kq + q];
+      if (!p.backward) {
+        if (live) *(float4 *)&p.out[(int64_t)row * NOUT + kq] = make_float4(y[0], y[1], y[2], y[3]);
+      } else {
+        // dY = 2 (y - t) / (n 8), the mean-squared error's gradient
+        floats<4>::type dy;
+        bf4 dv, zero;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const float d = live ? y[q] - tgt[col * NOUT + kq + q] : 0.0f;
+          sq += d * d;
+          dy[q] = sc * d;
+          dv[q] = (__bf16)dy[q];
+          zero[q] = (__bf16)0.0f;
+          Dt[(DT_OFF[3] + kq + q) * TP + col] = dv[q];
+        }
+        *(bf4 *)&act[0][col * AP + kq] = dv;
+        *(bf4 *)&act[0][col * AP + NOUT + kq] = zero;
+        int vi;
+        const float s = half_sums<4>(dy, lane, vi);
+        if ((lane & 7) == 0) part_add(p, PB_OFF + 3 * HID + kq + vi, s, first);
+      }
+    }
+    if (!p.backward) continue;
+    lds_barrier();
+    // ---- backward: layers 3, 2, 1, each beside the weight gradients of
+    // the layer above, whose dZ the last barrier published (independent
+    // products: the matrix cores run them while the epilogues run)
+    weight_grad_blocks<3>(p, Ht, Dt, wave, first, lane, col, kh);
+    backward_layer<1>(p, v4, act[0], act[1], sg3, &Dt[DT_OFF[2] * TP], PB_OFF + 2 * HID, first, wave, lane, col, kh, kq);
+    lds_barrier();
+    weight_grad_blocks<2>(p, Ht, Dt, wave, first, lane, col, kh);
+    backward_layer<8>(p, v3, act[1], act[0], sg2, &Dt[DT_OFF[1] * TP], PB_OFF + HID, first, wave, lane, col, kh, kq);
+    lds_barrier();
+    weight_grad_blocks<1>(p, Ht, Dt, wave, first, lane, col, kh);
+    backward_layer<8>(p, v2, act[0], nullptr, sg1, &Dt[DT_OFF[0] * TP], PB_OFF, first, wave, lane, col, kh,
+                      kq);
+    lds_barrier();
+    weight_grad_blocks<0>(p, Ht, Dt, wave, first, lane, col, kh);
+  }
+  if (p.backward && wave == 0) {
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+    if (lane == 0) p.part[(int64_t)blockIdx.x * PSTRIDE + PLOSS] = sq;
   }
 }
 
-// normalised batch: x = (raw[idx] - mu) / sd, y likewise; rows past n are 0
-__global__ void gather_kernel(const float *__restrict__ xs, const float *__restrict__ ys,
-                              const int64_t *__restrict__ idx, int n, float *__restrict__ xb,
-                              float *__restrict__ yb, const float *__restrict__ norm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t s = idx[i];
-#pragma unroll
-  for (int c = 0; c < NIN; c++) xb[(int64_t)i * NIN + c] = (xs[s * NIN + c] - norm[c]) / norm[NIN + c];
-#pragma unroll
-  for (int c = 0; c < NOUT; c++)
-    yb[(int64_t)i * NOUT + c] = (ys[s * NOUT + c] - norm[2 * NIN + c]) / norm[2 * NIN + NOUT + c];
+__global__ void __launch_bounds__(64 * NWAVE) __attribute__((amdgpu_waves_per_eu(1, 1)))
+rows_kernel(RowsArgs p) {
+  __shared__ RowsLDS sh;
+  rows_pass(p, sh);
 }
 
-// mean-squared-error gradient dY = 2 (Y - T) / (n * NOUT), and the loss
-__global__ void mse_kernel(const float *__restrict__ Y, const float *__restrict__ T, int n,
-                           float *__restrict__ dY, float *__restrict__ loss) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  float l = 0.0f;
-  if (i < n * NOUT) {
-    const float d = Y[i] - T[i];
-    dY[i] = 2.0f * d / (float)(n * NOUT);
-    l = d * d / (float)(n * NOUT);
+// (layer, input, output) of weight k of the packed vector; false for a bias
+__device__ __forceinline__ bool weight_at(int k, int &l, int &i, int &j) {
+#pragma unroll
+  for (l = 0; l < 4; l++) {
+    if (k < LAYER_B[l]) {
+      const int e = k - LAYER_W[l];
+      i = e / LAYER_OUT[l];
+      j = e - i * LAYER_OUT[l];
+      return true;
+    }
+    if (k < LAYER_B[l] + LAYER_OUT[l]) return false;
   }
-  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
-  if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, l);
+  return false;
 }
 
-// Adam on the FP32 master parameters; writes the BF16 working copy
-__global__ void adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
-                            const float *__restrict__ g, uint16_t *__restrict__ wb, int n, float lr,
-                            float b1, float b2, float eps, float c1, float c2) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.0f - b1) * gi;
-  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  w[i] = wi;
-  wb[i] = f32_to_bf16(wi);
+// where parameter k (k = NPARAM: the loss) sits in a slot of partials
+__device__ __forceinline__ int partial_index(int k) {
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const int N = LAYER_OUT[l];
+    if (k < LAYER_B[l]) {
+      const int e = k - LAYER_W[l], i = e / N, j = e - i * N;
+      const int ib = i >> 5, jb = j >> 5, ii = i & 31;
+      const int bk = l == 0 ? jb : l == 1 ? 4 + 4 * ib + jb : l == 2 ? 20 + 4 * ib + jb : 36 + ib;
+      // D[8g + 4h + q][c] is register 4g + q of lane 32h + c
+      return bk * PBLK + (4 * (ii >> 3) + (ii & 3)) * 64 + 32 * ((ii >> 2) & 1) + (j & 31);
+    }
+    if (k < LAYER_B[l] + N) return PB_OFF + l * HID + (k - LAYER_B[l]);
+  }
+  return PLOSS;
 }
 
-__global__ void to_bf16_kernel(const float *__restrict__ w, uint16_t *__restrict__ wb, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) wb[i] = f32_to_bf16(w[i]);
+// the BF16 copies of weight (l, i, j)
+__device__ __forceinline__ void store_copies(uint16_t *wT, uint16_t *wB, int l, int i, int j, float wij) {
+  const uint16_t b = f32_to_bf16(wij);
+  wT[WT_OFF[l] + j * (l == 0 ? K1 : HID) + i] = b;
+  if (l > 0) wB[WB_OFF[l] + i * (l == 3 ? N4B : HID) + j] = b;
+}
+
+// parameter k's gradient from the partials, then its Adam step (or the
+// gradient itself); k = NPARAM: the loss.  Four consecutive threads take
+// one parameter: thread s sums partials s, s+4, ..., so that its loads are
+// all in flight at once, and the four sums are added pairwise (the same
+// bits in every thread of the four, whatever the order of arrival).
+__global__ void __launch_bounds__(256) update_kernel(UpdateArgs u) {
+  const int t = blockIdx.x * 256 + threadIdx.x, k = t >> 2, s4 = t & 3;
+  if (k > NPARAM) return;              // (whole quads)
+  const int loc = partial_index(k);
+  float s = 0.0f;
+#pragma unroll 16
+  for (int w = s4; w < u.npart; w += 4) s += u.part[(int64_t)w * PSTRIDE + loc];
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if (s4) return;
+  if (k == NPARAM) {
+    if (u.loss) *u.loss = s / (float)(u.n * NOUT);
+    return;
+  }
+  if (!u.adam) {
+    u.g[k] = s;
+    return;
+  }
+  const float mi = u.b1 * u.m[k] + (1.0f - u.b1) * s;
+  const float vi = u.b2 * u.v[k] + (1.0f - u.b2) * s * s;
+  const float wk = u.w[k] - u.lr * (mi / u.c1) / (sqrtf(vi / u.c2) + u.eps);
+  u.m[k] = mi;
+  u.v[k] = vi;
+  u.w[k] = wk;
+  int l, i, j;
+  if (weight_at(k, l, i, j)) store_copies(u.wT, u.wB, l, i, j, wk);
+}
+
+// the BF16 copies of every weight, from the FP32 parameters
+__global__ void copies_kernel(const float *__restrict__ w, uint16_t *__restrict__ wT, uint16_t *__restrict__ wB) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int l, i, j;
+  if (k < NPARAM && weight_at(k, l, i, j)) store_copies(wT, wB, l, i, j, w[k]);
 }
 
 }  // namespace
@@ -185,14 +563,14 @@ __global__ void to_bf16_kernel(const float *__restrict__ w, uint16_t *__restrict
 struct cable_mlp {
   int batch = 0;
   hipStream_t stream = nullptr;
-  // parameters: FP32 master, BF16 copy, gradient, Adam moments (NPARAM each)
+  // parameters: FP32 master, gradient, Adam moments (NPARAM each), and the
+  // BF16 copies in the two layouts the products read
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  uint16_t *wb = nullptr;
+  uint16_t *wT = nullptr, *wB = nullptr;
   float *norm = nullptr;  // mu_x[24], sd_x[24], mu_y[8], sd_y[8]
-  // activations of one batch
-  float *x = nullptr, *y = nullptr, *z[3] = {nullptr, nullptr, nullptr}, *h[3] = {nullptr, nullptr, nullptr};
-  float *out = nullptr, *dy = nullptr, *dh = nullptr, *dz = nullptr, *loss = nullptr;
-  float *part = nullptr;      // split-K partials of the weight gradients
+  // rows handed in by forward/gradient, the outputs, the loss
+  float *x = nullptr, *y = nullptr, *out = nullptr, *loss = nullptr;
+  float *part = nullptr;  // partial gradients, GMAX x PSTRIDE
   // training samples (device)
   float *xs = nullptr, *ys = nullptr;
   int64_t nsamples = 0;
@@ -206,84 +584,80 @@ struct cable_mlp {
 
 namespace {
 
-template <class TA, class TB>
-hipError_t gemm(hipStream_t s, const TA *A, int64_t sam, int64_t sak, const TB *B, int64_t sbk, int64_t sbn,
-                float *C, float *H, int M, int N, int K, const float *bias, const float *Z, int epi,
-                int nsplit = 1) {
-  const int kchunk = ((K + nsplit - 1) / nsplit + 7) / 8 * 8;
-  dim3 grid((M + 31) / 32, (N + 31) / 32, nsplit);
-  hipLaunchKernelGGL((gemm_kernel<TA, TB>), grid, dim3(64), 0, s, A, sam, sak, B, sbk, sbn, C, H, M, N, K, bias,
-                     Z, epi, kchunk);
-  return hipGetLastError();
+int groups(int n) { return std::min((n + RW - 1) / RW, GMAX); }
+
+RowsArgs rows_args(cable_mlp *m, int n, bool backward, const int64_t *idx) {
+  RowsArgs a;
+  memset(&a, 0, sizeof a);
+  a.xs = m->xs;
+  a.ys = m->ys;
+  a.idx = idx;
+  a.x = m->x;
+  a.y = m->y;
+  a.norm = m->norm;
+  a.w = m->w;
+  a.wT = m->wT;
+  a.wB = m->wB;
+  a.out = m->out;
+  a.part = m->part;
+  a.n = n;
+  a.backward = backward ? 1 : 0;
+  return a;
 }
 
-// weight gradients: K (the batch) split in chunks of KSPLIT_ROWS rows
-constexpr int KSPLIT_ROWS = 256;
+// lr < 0: store the gradient; else Adam at rate lr (c1, c2 for the next step)
+UpdateArgs update_args(cable_mlp *m, int n, float lr, float *loss) {
+  UpdateArgs u;
+  memset(&u, 0, sizeof u);
+  u.part = m->part;
+  u.npart = groups(n);
+  u.n = n;
+  u.w = m->w;
+  u.m = m->m;
+  u.v = m->v;
+  u.g = m->g;
+  u.wT = m->wT;
+  u.wB = m->wB;
+  u.loss = loss;
+  if (lr >= 0.0f) {
+    u.adam = 1;
+    u.lr = lr;
+    u.b1 = 0.9f;
+    u.b2 = 0.999f;
+    u.eps = 1e-8f;
+    u.c1 = 1.0f - powf(u.b1, (float)(m->adam_t + 1));
+    u.c2 = 1.0f - powf(u.b2, (float)(m->adam_t + 1));
+  }
+  return u;
+}
 
-// forward pass of n rows of m->x into m->out (pre-activations kept)
-int forward(cable_mlp *m, int n) {
-  const uint16_t *W[4];
-  const float *b[4];
-  for (int l = 0; l < 4; l++) {
-    W[l] = m->wb + LAYER_W[l];
-    b[l] = m->w + LAYER_B[l];
-  }
-  const float *in = m->x;
-  for (int l = 0; l < 3; l++) {
-    const int K = LAYER_IN[l], N = LAYER_OUT[l];
-    HIPCHECK(gemm(m->stream, in, (int64_t)K, (int64_t)1, W[l], (int64_t)N, (int64_t)1, m->z[l], m->h[l], n, N, K,
-                  b[l], (const float *)nullptr, EPI_BIAS_SILU));
-    in = m->h[l];
-  }
-  HIPCHECK(gemm(m->stream, in, (int64_t)HID, (int64_t)1, W[3], (int64_t)NOUT, (int64_t)1, m->out, (float *)nullptr,
-                n, NOUT, HID, b[3], (const float *)nullptr, EPI_BIAS));
+// one pass of rows_kernel over n rows: forward only into m->out, or with
+// the backward pass into the partials; idx: gather raw samples (else m->x, m->y)
+int rows(cable_mlp *m, int n, bool backward, const int64_t *idx) {
+  hipLaunchKernelGGL(rows_kernel, dim3(groups(n)), dim3(64 * NWAVE), 0, m->stream, rows_args(m, n, backward, idx));
+  HIPCHECK(hipGetLastError());
   return CABLE_MLP_OK;
 }
 
-// gradients of the batch loss w.r.t. every parameter, into m->g
-int backward(cable_mlp *m, int n, float *loss) {
-  HIPCHECK(hipMemsetAsync(loss, 0, sizeof(float), m->stream));
-  hipLaunchKernelGGL(mse_kernel, dim3((n * NOUT + 255) / 256), dim3(256), 0, m->stream, m->out, m->y, n, m->dy,
-                     loss);
+// after a backward pass over n rows: the gradient into m->g (lr < 0) or an
+// Adam step at rate lr; the loss into *loss (device, optional)
+int update(cable_mlp *m, int n, float lr, float *loss) {
+  const UpdateArgs u = update_args(m, n, lr, loss);
+  if (lr >= 0.0f) m->adam_t++;
+  hipLaunchKernelGGL(update_kernel, dim3(((NPARAM + 1) * 4 + 255) / 256), dim3(256), 0, m->stream, u);
   HIPCHECK(hipGetLastError());
-  // layer 4: dW4 = H3^T dY, db4 = colsum dY, dH3 = dY W4^T
-  const float *dcur = m->dy;
-  int Ncur = NOUT;
-  for (int l = 3; l >= 0; l--) {
-    const int K = LAYER_IN[l], N = LAYER_OUT[l];
-    const float *hin = l == 0 ? m->x : m->h[l - 1];
-    // dW[l] (K x N) = hin^T (K x n) . dcur (n x N), split along the batch
-    const int nsplit = (n + KSPLIT_ROWS - 1) / KSPLIT_ROWS;
-    HIPCHECK(gemm(m->stream, hin, (int64_t)1, (int64_t)K, dcur, (int64_t)N, (int64_t)1, m->part,
-                  (float *)nullptr, K, N, n, (const float *)nullptr, (const float *)nullptr, EPI_STORE, nsplit));
-    hipLaunchKernelGGL(reduce_kernel, dim3((K * N + 255) / 256), dim3(256), 0, m->stream, m->part, nsplit,
-                       (int64_t)K * N, m->g + LAYER_W[l]);
-    HIPCHECK(hipGetLastError());
-    // db[l] = column sums of dcur; the partials reuse m->part (nsplit x N <= its size)
-    hipLaunchKernelGGL(colsum_kernel, dim3((N + 31) / 32, nsplit), dim3(256), 0, m->stream, dcur, n, N,
-                       KSPLIT_ROWS, m->part);
-    hipLaunchKernelGGL(reduce_kernel, dim3((N + 255) / 256), dim3(256), 0, m->stream, m->part, nsplit, (int64_t)N,
-                       m->g + LAYER_B[l]);
-    HIPCHECK(hipGetLastError());
-    if (l == 0) break;
-    // dZ[l-1] = (dcur . W[l]^T) * silu'(Z[l-1]); W[l] is K x N row-major
-    float *dzn = (dcur == m->dz) ? m->dh : m->dz;
-    HIPCHECK(gemm(m->stream, dcur, (int64_t)N, (int64_t)1, m->wb + LAYER_W[l], (int64_t)1, (int64_t)N, dzn,
-                  (float *)nullptr, n, K, N, (const float *)nullptr, m->z[l - 1], EPI_SILU_GRAD));
-    dcur = dzn;
-    Ncur = K;
-  }
-  (void)Ncur;
   return CABLE_MLP_OK;
 }
 
-int adam(cable_mlp *m, float lr) {
-  m->adam_t++;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float c1 = 1.0f - powf(b1, (float)m->adam_t), c2 = 1.0f - powf(b2, (float)m->adam_t);
-  hipLaunchKernelGGL(adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, m->w, m->m, m->v, m->g,
-                     m->wb, NPARAM, lr, b1, b2, eps, c1, c2);
-  HIPCHECK(hipGetLastError());
+// room for the losses of nbatches batches
+int ensure_losses(cable_mlp *m, int64_t nbatches) {
+  if (nbatches > m->losscap) {
+    if (m->losses) HIPCHECK(hipFree(m->losses));
+    m->losses = nullptr;
+    m->losscap = 0;
+    HIPCHECK(hipMalloc(&m->losses, sizeof(float) * (size_t)nbatches));
+    m->losscap = nbatches;
+  }
   return CABLE_MLP_OK;
 }
 
@@ -312,20 +686,14 @@ int cable_mlp_create(cable_mlp **out, int batch) {
   al((void **)&m->g, sizeof(float) * NPARAM);
   al((void **)&m->m, sizeof(float) * NPARAM);
   al((void **)&m->v, sizeof(float) * NPARAM);
-  al((void **)&m->wb, sizeof(uint16_t) * NPARAM);
+  al((void **)&m->wT, sizeof(uint16_t) * WT_SIZE);   // the pads stay zero
+  al((void **)&m->wB, sizeof(uint16_t) * WB_SIZE);
   al((void **)&m->norm, sizeof(float) * NNORM);
   al((void **)&m->x, sizeof(float) * (size_t)batch * NIN);
   al((void **)&m->y, sizeof(float) * (size_t)batch * NOUT);
-  for (int l = 0; l < 3; l++) {
-    al((void **)&m->z[l], sizeof(float) * (size_t)batch * HID);
-    al((void **)&m->h[l], sizeof(float) * (size_t)batch * HID);
-  }
   al((void **)&m->out, sizeof(float) * (size_t)batch * NOUT);
-  al((void **)&m->dy, sizeof(float) * (size_t)batch * NOUT);
-  al((void **)&m->dh, sizeof(float) * (size_t)batch * HID);
-  al((void **)&m->dz, sizeof(float) * (size_t)batch * HID);
   al((void **)&m->loss, sizeof(float));
-  al((void **)&m->part, sizeof(float) * HID * HID * (size_t)((batch + KSPLIT_ROWS - 1) / KSPLIT_ROWS));
+  al((void **)&m->part, sizeof(float) * (size_t)GMAX * PSTRIDE);
   if (e != hipSuccess) {
     cable_mlp_destroy(m);
     return fail(CABLE_MLP_E_HIP, "cable_mlp_create: %s", hipGetErrorString(e));
@@ -341,8 +709,8 @@ int cable_mlp_create(cable_mlp **out, int batch) {
 
 int cable_mlp_destroy(cable_mlp *m) {
   if (!m) return CABLE_MLP_OK;
-  void *ps[] = {m->w, m->g, m->m, m->v, m->wb, m->norm, m->x, m->y, m->z[0], m->z[1], m->z[2], m->h[0], m->h[1],
-                m->h[2], m->out, m->dy, m->dh, m->dz, m->loss, m->idx, m->losses, m->part};
+  void *ps[] = {m->w, m->g, m->m, m->v, m->wT, m->wB, m->norm, m->x, m->y, m->out, m->loss, m->part, m->idx,
+                m->losses};
   for (void *p : ps)
     if (p) hipFree(p);
   if (m->own_samples) {
@@ -357,7 +725,7 @@ int cable_mlp_destroy(cable_mlp *m) {
 int cable_mlp_set_params(cable_mlp *m, const float *params) {
   if (!m || !params) return fail(CABLE_MLP_E_ARG, "cable_mlp_set_params: null argument");
   HIPCHECK(hipMemcpyAsync(m->w, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, m->stream));
-  hipLaunchKernelGGL(to_bf16_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, m->w, m->wb, NPARAM);
+  hipLaunchKernelGGL(copies_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, m->w, m->wT, m->wB);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemsetAsync(m->m, 0, sizeof(float) * NPARAM, m->stream));
   HIPCHECK(hipMemsetAsync(m->v, 0, sizeof(float) * NPARAM, m->stream));
@@ -394,7 +762,7 @@ int cable_mlp_set_norm(cable_mlp *m, const float *norm) {
 int cable_mlp_forward(cable_mlp *m, const float *x, int n, float *y) {
   if (!m || !x || !y || n <= 0 || n > m->batch) return fail(CABLE_MLP_E_ARG, "cable_mlp_forward: bad arguments");
   int r = copy_rows_in(m, x, m->x, n, NIN);
-  if (!r) r = forward(m, n);
+  if (!r) r = rows(m, n, false, nullptr);
   if (r) return r;
   HIPCHECK(hipMemcpyAsync(y, m->out, sizeof(float) * (size_t)n * NOUT, hipMemcpyDeviceToHost, m->stream));
   HIPCHECK(hipStreamSynchronize(m->stream));
@@ -405,8 +773,8 @@ int cable_mlp_gradient(cable_mlp *m, const float *x, const float *t, int n, floa
   if (!m || !x || !t || n <= 0 || n > m->batch) return fail(CABLE_MLP_E_ARG, "cable_mlp_gradient: bad arguments");
   int r = copy_rows_in(m, x, m->x, n, NIN);
   if (!r) r = copy_rows_in(m, t, m->y, n, NOUT);
-  if (!r) r = forward(m, n);
-  if (!r) r = backward(m, n, m->loss);
+  if (!r) r = rows(m, n, true, nullptr);
+  if (!r) r = update(m, n, -1.0f, m->loss);
   if (r) return r;
   if (loss) HIPCHECK(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
   HIPCHECK(hipStreamSynchronize(m->stream));
@@ -443,6 +811,36 @@ int cable_mlp_use_samples(cable_mlp *m, const float *xs_dev, const float *ys_dev
   return CABLE_MLP_OK;
 }
 
+int cable_mlp_shuffled_order(uint64_t seed, int64_t n, int64_t first, int64_t count, int64_t *out) {
+  if (n <= 0 || first < 0 || count < 0 || (count && !out))
+    return fail(CABLE_MLP_E_ARG, "cable_mlp_shuffled_order: bad arguments");
+  for (int64_t i = 0; i < count; i++) out[i] = shuffled_sample(seed, first + i, n);
+  return CABLE_MLP_OK;
+}
+
+int cable_mlp_train_shuffled(cable_mlp *m, uint64_t seed, int64_t first_batch, int64_t nbatches, float lr,
+                             float *losses) {
+  if (!m || first_batch < 0 || nbatches <= 0) return fail(CABLE_MLP_E_ARG, "cable_mlp_train_shuffled: bad arguments");
+  if (!m->xs) return fail(CABLE_MLP_E_STATE, "cable_mlp_train_shuffled: no samples loaded");
+  int r = ensure_losses(m, nbatches);
+  if (r) return r;
+  const int B = m->batch;
+  for (int64_t b = 0; b < nbatches; b++) {
+    RowsArgs a = rows_args(m, B, true, nullptr);
+    a.shuffled = 1;
+    a.seed = seed;
+    a.nsamples = m->nsamples;
+    a.pos0 = (first_batch + b) * B;
+    hipLaunchKernelGGL(rows_kernel, dim3(groups(B)), dim3(64 * NWAVE), 0, m->stream, a);
+    HIPCHECK(hipGetLastError());
+    r = update(m, B, lr, m->losses + b);
+    if (r) return r;
+  }
+  if (losses) HIPCHECK(hipMemcpyAsync(losses, m->losses, sizeof(float) * (size_t)nbatches, hipMemcpyDeviceToHost, m->stream));
+  HIPCHECK(hipStreamSynchronize(m->stream));
+  return CABLE_MLP_OK;
+}
+
 int cable_mlp_train(cable_mlp *m, const int64_t *order, int64_t nbatches, float lr, float *losses) {
   if (!m || !order || nbatches <= 0) return fail(CABLE_MLP_E_ARG, "cable_mlp_train: bad arguments");
   if (!m->xs) return fail(CABLE_MLP_E_STATE, "cable_mlp_train: no samples loaded");
@@ -458,21 +856,14 @@ int cable_mlp_train(cable_mlp *m, const int64_t *order, int64_t nbatches, float 
     HIPCHECK(hipMalloc(&m->idx, sizeof(int64_t) * (size_t)total));
     m->idxcap = total;
   }
-  if (nbatches > m->losscap) {
-    if (m->losses) HIPCHECK(hipFree(m->losses));
-    m->losses = nullptr;
-    m->losscap = 0;
-    HIPCHECK(hipMalloc(&m->losses, sizeof(float) * (size_t)nbatches));
-    m->losscap = nbatches;
-  }
+  int r0 = ensure_losses(m, nbatches);
+  if (r0) return r0;
   HIPCHECK(hipMemcpyAsync(m->idx, order, sizeof(int64_t) * (size_t)total, hipMemcpyHostToDevice, m->stream));
+  // two launches per batch.  (One cooperative launch for all batches, with
+  // grid barriers between the passes, measured three times slower.)
   for (int64_t b = 0; b < nbatches; b++) {
-    hipLaunchKernelGGL(gather_kernel, dim3((B + 255) / 256), dim3(256), 0, m->stream, m->xs, m->ys, m->idx + b * B, B,
-                       m->x, m->y, m->norm);
-    HIPCHECK(hipGetLastError());
-    int r = forward(m, B);
-    if (!r) r = backward(m, B, m->losses + b);
-    if (!r) r = adam(m, lr);
+    int r = rows(m, B, true, m->idx + b * B);
+    if (!r) r = update(m, B, lr, m->losses + b);
     if (r) return r;
   }
   if (losses) HIPCHECK(hipMemcpyAsync(losses, m->losses, sizeof(float) * (size_t)nbatches, hipMemcpyDeviceToHost, m->stream));

@@ -41,6 +41,43 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 __device__ inline float to_f32(float x) { return x; }
 __device__ inline float to_f32(uint16_t x) { return bf16_to_f32(x); }
 
+// The training order: an endless sequence of epochs, each a permutation of
+// the n samples chosen by (seed, epoch).  The permutation is a keyed
+// bijection of [0, 2^bits) (2^bits the power of two at or above n) -- four
+// rounds of multiply by an odd key, xor-shift right and add a key, each
+// invertible modulo 2^bits -- restricted to [0, n) by cycle walking: an
+// image at or above n is mapped again, and the walk ends inside [0, n)
+// because it follows a cycle through the starting point.  Each step leaves
+// [0, n) with probability at most 1/2, so the cap of 4096 steps is never
+// reached in practice; it only keeps the loop finite.  The keys come from
+// the splitmix64 finaliser (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014).
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline int64_t shuffled_sample(uint64_t seed, int64_t pos, int64_t n) {
+  const int64_t epoch = pos / n, i = pos - epoch * n;
+  int bits = 1;
+  while (bits < 62 && (int64_t(1) << bits) < n) bits++;
+  const uint64_t mask = (uint64_t(1) << bits) - 1;
+  const int sh = (bits + 1) / 2;
+  uint64_t key[4];
+  for (int r = 0; r < 4; r++) key[r] = mix64(seed ^ mix64((uint64_t)epoch * 4 + r));
+  uint64_t x = (uint64_t)i;
+  for (int step = 0; step < 4096; step++) {
+    for (int r = 0; r < 4; r++) {
+      x = (x * (key[r] | 1)) & mask;
+      x ^= x >> sh;
+      x = (x + (key[r] >> 32)) & mask;
+    }
+    if (x < (uint64_t)n) return (int64_t)x;
+  }
+  return i;
+}
+
 __device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
 __device__ inline float silu_grad(float z) {
   const float s = 1.0f / (1.0f + __expf(-z));

@@ -37,6 +37,8 @@ def lib():
             "cable_mlp_load_samples": ([vp, vp, vp, i64], c_int),
             "cable_mlp_use_samples": ([vp, vp, vp, i64], c_int),
             "cable_mlp_train": ([vp, vp, i64, c_float, vp], c_int),
+            "cable_mlp_train_shuffled": ([vp, ctypes.c_uint64, i64, i64, c_float, vp], c_int),
+            "cable_mlp_shuffled_order": ([ctypes.c_uint64, i64, i64, i64, vp], c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -148,24 +150,39 @@ class MLP:
     def train(self, epochs=1.0, lr=1e-3, seed=0, lr_final=None):
         """Adam over shuffled batches of the loaded samples for ``epochs``
         passes; the learning rate decays linearly to lr_final (default lr/10)
-        in chunks.  Returns the loss of every batch."""
-        rng = np.random.default_rng(seed)
+        in chunks.  The device draws the order itself (each epoch a
+        permutation keyed by seed and epoch; see shuffled_order).  Returns
+        the loss of every batch."""
         n, B = self.nsamples, self.batch
         nb = max(1, int(round(epochs * n / B)))
         lr_final = lr / 10.0 if lr_final is None else lr_final
         losses = np.zeros(nb, dtype=np.float32)
         done, chunk = 0, 512
-        order = np.empty(0, dtype=np.int64)
         while done < nb:
             k = min(chunk, nb - done)
-            while order.size < k * B:
-                order = np.concatenate([order, rng.permutation(n).astype(np.int64)])
-            ob, order = np.ascontiguousarray(order[:k * B]), order[k * B:]
             rate = lr + (lr_final - lr) * (done + 0.5 * k) / nb
-            _check(lib().cable_mlp_train(self.h, ob.ctypes.data, k, float(rate),
-                                         losses[done:].ctypes.data))
+            _check(lib().cable_mlp_train_shuffled(self.h, int(seed), done, k, float(rate),
+                                                  losses[done:].ctypes.data))
             done += k
         return losses
+
+    def train_order(self, order, lr):
+        """Adam steps over batches in a given order of sample indices
+        (len(order) a multiple of the batch).  Returns the batch losses."""
+        order = np.ascontiguousarray(order, dtype=np.int64)
+        k = order.size // self.batch
+        losses = np.zeros(k, dtype=np.float32)
+        _check(lib().cable_mlp_train(self.h, order.ctypes.data, k, float(lr), losses.ctypes.data))
+        return losses
+
+
+def shuffled_order(seed, n, first, count):
+    """Sample indices at positions first .. first+count-1 of the training
+    order over n samples (epoch e = position // n is a permutation of
+    range(n) keyed by seed and e)."""
+    out = np.empty(int(count), dtype=np.int64)
+    _check(lib().cable_mlp_shuffled_order(int(seed), int(n), int(first), int(count), out.ctypes.data))
+    return out
 
 
 def capture_samples(n_tiles=262144, nsteps=64, seed=None, order=True):

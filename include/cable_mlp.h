@@ -61,6 +61,18 @@ int cable_mlp_use_samples(cable_mlp *m, const float *x_dev, const float *y_dev, 
    losses[b] (optional) receives each batch's loss */
 int cable_mlp_train(cable_mlp *m, const int64_t *order, int64_t nbatches, float lr, float *losses);
 
+/* nbatches Adam steps in an order the device draws itself: batch b takes
+   positions (first_batch + b) * batch .. + batch - 1 of an endless sequence
+   of epochs, each a permutation of the loaded samples keyed by (seed,
+   epoch); losses[b] (optional) receives each batch's loss.  The same steps
+   as cable_mlp_train with the order cable_mlp_shuffled_order gives, with no
+   order to compute on the host or copy to the device. */
+int cable_mlp_train_shuffled(cable_mlp *m, uint64_t seed, int64_t first_batch, int64_t nbatches, float lr,
+                             float *losses);
+/* the sample indices at positions first .. first + count - 1 of that
+   sequence over n samples (host only) */
+int cable_mlp_shuffled_order(uint64_t seed, int64_t n, int64_t first, int64_t count, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

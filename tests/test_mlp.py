@@ -40,6 +40,25 @@ def test_oracle_adam_matches_closed_form_first_step():
     np.testing.assert_allclose(p, -0.01 * np.sign(g), rtol=1e-4)
 
 
+def test_shuffled_order_is_a_permutation_of_every_epoch():
+    """The order the device draws for training (cable_mlp_shuffled_order,
+    the same function the rows kernel calls): every epoch of n positions is
+    a permutation of range(n), for n at and around powers of two (where the
+    cycle walk has the most to skip) and n = 1; consecutive epochs and
+    different seeds give different orders, and few samples stay in place."""
+    from cable_amd.surrogate import shuffled_order
+    for n in (1, 2, 3, 5, 64, 65, 1000, 4096, 4097, 65543):
+        two = shuffled_order(7, n, 0, 2 * n)
+        for e in range(2):
+            np.testing.assert_array_equal(np.sort(two[e * n:(e + 1) * n]), np.arange(n))
+        if n >= 1000:
+            assert (two[:n] != two[n:]).mean() > 0.99
+            assert (shuffled_order(8, n, 0, n) != two[:n]).mean() > 0.99
+            assert (two[:n] == np.arange(n)).mean() < 0.01
+    # a window starting mid-epoch is the same sequence
+    np.testing.assert_array_equal(shuffled_order(7, 1000, 1500, 700), shuffled_order(7, 1000, 0, 2200)[1500:])
+
+
 def _layer_slices():
     o, out = 0, []
     for i, n in O.SHAPES:
@@ -124,3 +143,31 @@ def test_gpu_training_fits_a_smooth_map():
     print(f"loss: first {losses[:8].mean():.4f} last {losses[-8:].mean():.5f}", flush=True)
     assert losses[-8:].mean() < 0.05 * losses[:8].mean()
     net.close()
+
+
+@pytest.mark.gpu
+def test_gpu_shuffled_training_is_training_in_that_order():
+    """cable_mlp_train_shuffled draws the order on the device; the same
+    batches through cable_mlp_train with the host's copy of that order give
+    the same parameters and losses bit for bit (here across an epoch
+    boundary: positions 3,584 .. 5,631 of 5,000 samples)."""
+    from cable_amd.surrogate import MLP, shuffled_order
+    print("[mlp] shuffled order", flush=True)
+    rng = np.random.default_rng(8)
+    p = O.init_params(9).astype(np.float32)
+    x = rng.normal(size=(5000, O.NIN)).astype(np.float32)
+    y = rng.normal(size=(5000, O.NOUT)).astype(np.float32)
+    out = []
+    for device_order in (True, False):
+        net = MLP(batch=512, params=p)
+        net.load_samples(x, y)
+        if device_order:
+            losses = np.zeros(4, dtype=np.float32)
+            from cable_amd import surrogate as S
+            S._check(S.lib().cable_mlp_train_shuffled(net.h, 11, 7, 4, 1e-3, losses.ctypes.data))
+        else:
+            losses = net.train_order(shuffled_order(11, 5000, 7 * 512, 4 * 512), 1e-3)
+        out.append((net.params(), losses))
+        net.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
