@@ -14,6 +14,9 @@
 #include <string>
 #include <mutex>
 #include <vector>
+#include <thread>
+#include <atomic>
+#include <algorithm>
 
 #include "../../include/cable_gpu.h"
 #include "cbm_kernel.h"
@@ -143,11 +146,164 @@ struct cable_gpu_ctx {
   bool cons_on = false;
   double *cons_part = nullptr;      // per-workgroup sums of a step
   double *cons_acc = nullptr;       // count_bal, new_sumbal, new_sumfpn, new_sumfe
+  // ---- batched transfers: a pinned host staging buffer and its device
+  // twin ([transfer table | fields, each at a 256-byte offset]) ----
+  unsigned char *h_stage = nullptr, *d_stage = nullptr;
+  size_t stage_bytes = 0;
 };
+
+namespace {
+// ---- batched transfers ----
+// One entry per field of a batched transfer: the field's device array, its
+// offset in the staging buffer and its length in 4-byte words.
+struct XferEntry {
+  void *dev;
+  unsigned long long off, words;
+};
+constexpr size_t XFER_TABLE_BYTES = sizeof(XferEntry) * CABLE_NFIELDS;
+constexpr size_t XFER_ALIGN = 256;
+
+// stage -> fields (upload) or fields -> stage (download); blockIdx.y is the
+// entry, the x blocks stride over its words
+__global__ void xfer_kernel(const XferEntry *__restrict__ tab, uint32_t *__restrict__ stage, int to_fields) {
+  const XferEntry e = tab[blockIdx.y];
+  uint32_t *st = stage + e.off / 4, *dv = (uint32_t *)e.dev;
+  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < e.words;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    if (to_fields)
+      dv[i] = st[i];
+    else
+      st[i] = dv[i];
+  }
+}
+
+// host values of one field <-> its device-typed image in the staging
+// buffer, elements [i0, i1) (tile order and type conversion applied)
+void pack_range(const cable_gpu_ctx *c, const FieldMeta &f, const void *host, int ht, unsigned char *img, size_t i0,
+                size_t i1, bool to_device) {
+  const size_t de = dev_elem(f, c->precision), mp = (size_t)c->mp;
+  const int64_t *ord = c->order.empty() ? nullptr : c->order.data();
+  const bool same = (f.prec == 'I' && ht == CABLE_HOST_I32) || (f.prec != 'I' && de == 8 && ht == CABLE_HOST_F64) ||
+                    (f.prec != 'I' && de == 4 && ht == CABLE_HOST_F32);
+  if (same && !ord) {
+    if (to_device)
+      memcpy(img + i0 * de, (const unsigned char *)host + i0 * de, (i1 - i0) * de);
+    else
+      memcpy((unsigned char *)host + i0 * de, img + i0 * de, (i1 - i0) * de);
+    return;
+  }
+  for (size_t i = i0; i < i1; i++) {
+    // rows of mp tiles; device position j of a row holds caller tile ord[j]
+    const size_t h = ord ? (i - i % mp) + (size_t)ord[i % mp] : i;
+    if (to_device) {
+      const double v = ht == CABLE_HOST_F64 ? ((const double *)host)[h]
+                       : ht == CABLE_HOST_F32 ? (double)((const float *)host)[h]
+                                              : (double)((const int32_t *)host)[h];
+      if (f.prec == 'I')
+        ((int32_t *)img)[i] = (int32_t)llround(v);
+      else if (de == 8)
+        ((double *)img)[i] = v;
+      else
+        ((float *)img)[i] = (float)v;
+    } else {
+      const double v = f.prec == 'I' ? (double)((const int32_t *)img)[i]
+                       : de == 8     ? ((const double *)img)[i]
+                                     : (double)((const float *)img)[i];
+      if (ht == CABLE_HOST_F64)
+        ((double *)host)[h] = v;
+      else if (ht == CABLE_HOST_F32)
+        ((float *)host)[h] = (float)v;
+      else
+        ((int32_t *)host)[h] = (int32_t)llround(v);
+    }
+  }
+}
+
+int batched_transfer(cable_gpu_ctx *c, int n, const int *ids, void *const *hosts, const int *types, bool up);
+}  // namespace
 
 namespace {
 size_t field_bytes(const cable_gpu_ctx *c, int id) {
   return dev_elem(kFields[id], c->precision) * (size_t)kFields[id].n * (size_t)c->mp;
+}
+
+// Several fields in one copy each way: the host side packs (unpacks) the
+// fields' device-typed images in the pinned staging buffer on up to eight
+// threads, one copy moves the buffer, one kernel scatters (gathers) the
+// fields on the device.  The transfer table rides in front of the upload's
+// data; a download sends it ahead in a copy of its own.
+int batched_transfer(cable_gpu_ctx *c, int n, const int *ids, void *const *hosts, const int *types, bool up) {
+  const char *who = up ? "cable_gpu_upload_fields" : "cable_gpu_download_fields";
+  if (n > CABLE_NFIELDS) return fail(CABLE_E_ARG, "%s: %d fields (at most %d)", who, n, CABLE_NFIELDS);
+  std::vector<XferEntry> tab(n);
+  size_t off = XFER_TABLE_BYTES, maxw = 0;
+  for (int k = 0; k < n; k++) {
+    if (ids[k] < 0 || ids[k] >= CABLE_NFIELDS || !hosts[k])
+      return fail(CABLE_E_ARG, "%s: bad field %d (entry %d)", who, ids[k], k);
+    if (types[k] != CABLE_HOST_F64 && types[k] != CABLE_HOST_F32 && types[k] != CABLE_HOST_I32)
+      return fail(CABLE_E_ARG, "%s: bad host type %d (entry %d)", who, types[k], k);
+    const size_t b = field_bytes(c, ids[k]);
+    tab[k] = {c->dev[ids[k]], off, b / 4};
+    maxw = std::max(maxw, (size_t)(b / 4));
+    off += (b + XFER_ALIGN - 1) / XFER_ALIGN * XFER_ALIGN;
+  }
+  if (n == 0) return CABLE_OK;
+  if (off > c->stage_bytes) {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->h_stage) HIPCHECK(hipHostFree(c->h_stage));
+    if (c->d_stage) HIPCHECK(hipFree(c->d_stage));
+    c->h_stage = c->d_stage = nullptr;
+    c->stage_bytes = 0;
+    HIPCHECK(hipHostMalloc((void **)&c->h_stage, off, hipHostMallocDefault));
+    HIPCHECK(hipMalloc((void **)&c->d_stage, off));
+    c->stage_bytes = off;
+  }
+  memcpy(c->h_stage, tab.data(), sizeof(XferEntry) * n);
+  // the fields cut in pieces of at most 4 Mi elements, shared out to threads
+  struct Piece {
+    int k;
+    size_t i0, i1;
+  };
+  std::vector<Piece> pieces;
+  for (int k = 0; k < n; k++) {
+    const size_t cnt = (size_t)kFields[ids[k]].n * (size_t)c->mp, step = (size_t)1 << 22;
+    for (size_t i = 0; i < cnt; i += step) pieces.push_back({k, i, std::min(cnt, i + step)});
+  }
+  auto work = [&](bool to_device) {
+    std::atomic<size_t> next(0);
+    auto run = [&]() {
+      for (size_t q; (q = next.fetch_add(1)) < pieces.size();) {
+        const Piece &pc = pieces[q];
+        pack_range(c, kFields[ids[pc.k]], hosts[pc.k], types[pc.k], c->h_stage + tab[pc.k].off, pc.i0, pc.i1,
+                   to_device);
+      }
+    };
+    const unsigned nt = (unsigned)std::min<size_t>({8, pieces.size(), std::max(1u, std::thread::hardware_concurrency())});
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(run);
+    run();
+    for (auto &t : th) t.join();
+  };
+  const dim3 grid((unsigned)std::min<size_t>(1024, (maxw + 255) / 256), (unsigned)n);
+  if (up) {
+    c->touched = true;
+    work(true);
+    HIPCHECK(hipMemcpyAsync(c->d_stage, c->h_stage, off, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(xfer_kernel, grid, dim3(256), 0, c->stream, (const XferEntry *)c->d_stage,
+                       (uint32_t *)c->d_stage, 1);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  } else {
+    HIPCHECK(hipMemcpyAsync(c->d_stage, c->h_stage, sizeof(XferEntry) * n, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(xfer_kernel, grid, dim3(256), 0, c->stream, (const XferEntry *)c->d_stage,
+                       (uint32_t *)c->d_stage, 0);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(c->h_stage + XFER_TABLE_BYTES, c->d_stage + XFER_TABLE_BYTES, off - XFER_TABLE_BYTES,
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    work(false);
+  }
+  return CABLE_OK;
 }
 
 StepArgs make_args(const cable_gpu_ctx *c, void **tab, int ktau, double dels, int64_t foff) {
@@ -307,6 +463,8 @@ int cable_gpu_destroy(cable_gpu_ctx *c) {
                  c->nfallback};
   for (void *p : sur)
     if (p) hipFree(p);
+  if (c->h_stage) hipHostFree(c->h_stage);
+  if (c->d_stage) hipFree(c->d_stage);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   for (hipEvent_t ev : c->kev) hipEventDestroy(ev);
@@ -348,6 +506,19 @@ int cable_gpu_download(cable_gpu_ctx *c, int id, void *host, int host_type) {
     }
   }
   return CABLE_OK;
+}
+
+int cable_gpu_upload_fields(cable_gpu_ctx *c, int n, const int *ids, const void *const *hosts,
+                            const int *host_types) {
+  if (!c || n < 0 || (n && (!ids || !hosts || !host_types)))
+    return fail(CABLE_E_ARG, "cable_gpu_upload_fields: bad arguments");
+  return batched_transfer(c, n, ids, const_cast<void *const *>(hosts), host_types, true);
+}
+
+int cable_gpu_download_fields(cable_gpu_ctx *c, int n, const int *ids, void *const *hosts, const int *host_types) {
+  if (!c || n < 0 || (n && (!ids || !hosts || !host_types)))
+    return fail(CABLE_E_ARG, "cable_gpu_download_fields: bad arguments");
+  return batched_transfer(c, n, ids, hosts, host_types, false);
 }
 
 // ---------------------------------------------------------------------------

@@ -9,6 +9,7 @@ USE, INTRINSIC :: ISO_C_BINDING
 IMPLICIT NONE
 PRIVATE
 PUBLIC :: cable_gpu_create, cable_gpu_destroy, cable_gpu_upload, cable_gpu_download, &
+          cable_gpu_upload_fields, cable_gpu_download_fields, cable_gpu_field_id, &
           cable_gpu_set_globals, cable_gpu_cbm, cable_gpu_step, cable_gpu_sync, &
           cable_gpu_check, cable_gpu_error_message
 ! the device-resident run (a driver that owns its time loop)
@@ -58,6 +59,29 @@ INTERFACE
     TYPE(C_PTR), VALUE :: ctx, host
     INTEGER(C_INT), VALUE :: field_id, host_type
   END FUNCTION cable_gpu_download
+
+  INTEGER(C_INT) FUNCTION cable_gpu_field_id(name) BIND(C, NAME='cable_gpu_field_id')
+    IMPORT :: C_INT, C_CHAR
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: name(*)
+  END FUNCTION cable_gpu_field_id
+
+  INTEGER(C_INT) FUNCTION cable_gpu_upload_fields(ctx, n, field_ids, hosts, host_types) &
+      BIND(C, NAME='cable_gpu_upload_fields')
+    IMPORT :: C_PTR, C_INT
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: n
+    INTEGER(C_INT), INTENT(IN) :: field_ids(*), host_types(*)
+    TYPE(C_PTR), INTENT(IN) :: hosts(*)
+  END FUNCTION cable_gpu_upload_fields
+
+  INTEGER(C_INT) FUNCTION cable_gpu_download_fields(ctx, n, field_ids, hosts, host_types) &
+      BIND(C, NAME='cable_gpu_download_fields')
+    IMPORT :: C_PTR, C_INT
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: n
+    INTEGER(C_INT), INTENT(IN) :: field_ids(*), host_types(*)
+    TYPE(C_PTR), INTENT(IN) :: hosts(*)
+  END FUNCTION cable_gpu_download_fields
 
   INTEGER(C_INT) FUNCTION cable_gpu_set_globals(ctx, zse, zshh, ratecp, ratecs) &
       BIND(C, NAME='cable_gpu_set_globals')

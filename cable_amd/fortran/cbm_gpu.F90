@@ -24,6 +24,18 @@
 ! src/science/soilsnow/cbl_snowAccum.F90), and those tests flip when an FP64
 ! value is rounded to FP32 between steps.
 !
+! Transfers.  By default every call uploads every field the step reads and
+! downloads every field it writes, so the host's derived types are the
+! state, as the reference's are.  With cable_gpu_transfers = 'resident' (or the
+! environment variable CABLE_GPU_SYNC=resident) the device holds the state:
+! after the first call only the components the offline driver writes between
+! calls go up (the forcing, canopy%oldcansto, veg%vlai;
+! src/offline/cable_serial.F90:572-575), and a host that reads back only some
+! fields lists their ids in cable_gpu_download_only (or their names, comma
+! separated, in CABLE_GPU_DOWNLOAD); canopy%cansto always comes back, since
+! the driver copies it into oldcansto.  Each direction is one batched
+! transfer through a pinned staging buffer (cable_gpu_upload_fields).
+!
 ! The device implements the shipped configuration (src/offline/cable.nml):
 ! icycle = 0 and the cable_user switches the path's science branches on at
 ! their shipped values (soil_struc 'default', fwsoil_switch 'standard',
@@ -38,6 +50,9 @@ IMPLICIT NONE
 PRIVATE
 PUBLIC :: cbm, cbm_gpu_finalise
 INTEGER(C_INT), PUBLIC, SAVE :: cable_gpu_precision = -1   ! -1: from KIND(1.0)
+CHARACTER(LEN=16), PUBLIC, SAVE :: cable_gpu_transfers = ''      ! 'all' or 'resident'; blank: CABLE_GPU_SYNC or 'all'
+INTEGER(C_INT), ALLOCATABLE, PUBLIC, SAVE :: cable_gpu_download_only(:)   ! field ids (resident only)
+LOGICAL, SAVE :: resident = .FALSE.
 
 TYPE(C_PTR), SAVE :: ctx = C_NULL_PTR
 
@@ -47,7 +62,7 @@ SUBROUTINE cbm(ktau, dels, air, bgc, canopy, met, bal, rad, rough, soil, &
                ssnow, sum_flux, veg, climate, xk, c1, rhoch)
   USE cable_def_types_mod
   USE cable_climate_type_mod, ONLY: climate_type
-  USE cable_gpu_fields, ONLY: cable_gpu_put_fields, cable_gpu_get_fields
+  USE cable_gpu_fields, ONLY: cable_gpu_put_fields, cable_gpu_get_fields, CABLE_F_canopy_cansto
 
   TYPE (air_type),       INTENT(INOUT) :: air
   TYPE (bgc_pool_type),  INTENT(INOUT) :: bgc
@@ -68,9 +83,12 @@ SUBROUTINE cbm(ktau, dels, air, bgc, canopy, met, bal, rad, rough, soil, &
   REAL :: xk(mp,nrb)
 
   REAL(C_DOUBLE) :: zse(ms), zshh(ms+1), ratecp(ncp), ratecs(ncs)
+  LOGICAL :: first
 
-  IF (.NOT. C_ASSOCIATED(ctx)) THEN
+  first = .NOT. C_ASSOCIATED(ctx)
+  IF (first) THEN
     CALL check_configuration()
+    CALL transfer_options()
     IF (cable_gpu_precision < 0) &
       cable_gpu_precision = MERGE(CABLE_PREC_FP64, CABLE_PREC_MIXED, KIND(1.0) == 8)
     CALL cable_gpu_check(cable_gpu_create(ctx, INT(mp, C_INT64_T), cable_gpu_precision))
@@ -80,10 +98,47 @@ SUBROUTINE cbm(ktau, dels, air, bgc, canopy, met, bal, rad, rough, soil, &
     ratecs = bgc%ratecs
     CALL cable_gpu_check(cable_gpu_set_globals(ctx, zse, zshh, ratecp, ratecs))
   END IF
-  CALL cable_gpu_put_fields(ctx, air, bgc, canopy, met, bal, rad, rough, soil, ssnow, veg)
+  CALL cable_gpu_put_fields(ctx, air, bgc, canopy, met, bal, rad, rough, soil, ssnow, veg, &
+                            MERGE(1, 0, resident .AND. .NOT. first))
   CALL cable_gpu_check(cable_gpu_cbm(ctx, INT(ktau, C_INT), REAL(dels, C_DOUBLE)))
-  CALL cable_gpu_get_fields(ctx, air, bgc, canopy, met, bal, rad, rough, soil, ssnow, veg)
+  IF (ALLOCATED(cable_gpu_download_only)) THEN
+    CALL cable_gpu_get_fields(ctx, air, bgc, canopy, met, bal, rad, rough, soil, ssnow, veg, &
+                              [cable_gpu_download_only, CABLE_F_canopy_cansto])
+  ELSE
+    CALL cable_gpu_get_fields(ctx, air, bgc, canopy, met, bal, rad, rough, soil, ssnow, veg)
+  END IF
 END SUBROUTINE cbm
+
+! cable_gpu_transfers and the download list, from the environment when the host
+! has not set them
+SUBROUTINE transfer_options()
+  CHARACTER(LEN=4096) :: names
+  INTEGER :: st, i, j, k
+  IF (LEN_TRIM(cable_gpu_transfers) == 0) THEN
+    CALL GET_ENVIRONMENT_VARIABLE('CABLE_GPU_SYNC', cable_gpu_transfers, STATUS=st)
+    IF (st /= 0) cable_gpu_transfers = 'all'
+  END IF
+  CALL need(TRIM(cable_gpu_transfers) == 'all' .OR. TRIM(cable_gpu_transfers) == 'resident', &
+            "cable_gpu_transfers = 'all' or 'resident'")
+  resident = TRIM(cable_gpu_transfers) == 'resident'
+  IF (.NOT. ALLOCATED(cable_gpu_download_only)) THEN
+    CALL GET_ENVIRONMENT_VARIABLE('CABLE_GPU_DOWNLOAD', names, STATUS=st)
+    IF (st == 0 .AND. LEN_TRIM(names) > 0) THEN
+      ALLOCATE(cable_gpu_download_only(0))
+      i = 1
+      DO WHILE (i <= LEN_TRIM(names))
+        j = INDEX(names(i:), ',')
+        j = MERGE(LEN_TRIM(names), i + j - 2, j == 0)
+        k = cable_gpu_field_id(TRIM(ADJUSTL(names(i:j))) // C_NULL_CHAR)
+        CALL need(k >= 0, 'a field name in CABLE_GPU_DOWNLOAD: ' // TRIM(ADJUSTL(names(i:j))))
+        cable_gpu_download_only = [cable_gpu_download_only, INT(k, C_INT)]
+        i = j + 2
+      END DO
+    END IF
+  END IF
+  CALL need(resident .OR. .NOT. ALLOCATED(cable_gpu_download_only), &
+            "cable_gpu_transfers = 'resident' for a download list (each call would upload the rest from the host)")
+END SUBROUTINE transfer_options
 
 ! The run's configuration against the one the device implements; stops on
 ! the first difference (before any device call).

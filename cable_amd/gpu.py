@@ -27,7 +27,8 @@ HOST_F64, HOST_F32, HOST_I32 = 0, 1, 2
 EXPORTS = [
     "cable_gpu_nfields", "cable_gpu_field_info", "cable_gpu_field_id",
     "cable_gpu_create", "cable_gpu_destroy", "cable_gpu_upload",
-    "cable_gpu_download", "cable_gpu_set_globals", "cable_gpu_step", "cable_gpu_cbm",
+    "cable_gpu_download", "cable_gpu_upload_fields", "cable_gpu_download_fields", "cable_gpu_set_globals",
+    "cable_gpu_step", "cable_gpu_cbm",
     "cable_gpu_veg_cols", "cable_gpu_soil_cols", "cable_gpu_table_column",
     "cable_gpu_set_class_tables", "cable_gpu_set_forcing", "cable_gpu_run",
     "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_last_kernel_ms", "cable_gpu_counters",
@@ -75,6 +76,8 @@ def lib():
         "cable_gpu_destroy": ([vp], c_int),
         "cable_gpu_upload": ([vp, c_int, vp, c_int], c_int),
         "cable_gpu_download": ([vp, c_int, vp, c_int], c_int),
+        "cable_gpu_upload_fields": ([vp, c_int, vp, vp, vp], c_int),
+        "cable_gpu_download_fields": ([vp, c_int, vp, vp, vp], c_int),
         "cable_gpu_set_globals": ([vp, vp, vp, vp, vp], c_int),
         "cable_gpu_step": ([vp, c_int, c_double], c_int),
         "cable_gpu_cbm": ([vp, c_int, c_double], c_int),
@@ -195,6 +198,35 @@ class CableGPU:
         out = np.empty(f.shape(self.mp), dtype=np.float64)
         _check(lib().cable_gpu_download(self.h, f.fid, _ptr(out), HOST_F64))
         return out
+
+    def upload_fields(self, fields: dict):
+        """Several fields in one batched transfer (cable_gpu_upload_fields);
+        fields: {name: array} (float64)."""
+        names = [k for k in fields if k in BY_NAME]
+        arrs = []
+        for k in names:
+            a = np.ascontiguousarray(fields[k], dtype=np.float64).reshape(-1)
+            if a.size != BY_NAME[k].n * self.mp:
+                raise CableError(f"{k}: {a.size} elements, expected {BY_NAME[k].n * self.mp}")
+            arrs.append(a)
+        self._batched(lib().cable_gpu_upload_fields, names, arrs)
+
+    def download_fields(self, names, out=None):
+        """Several fields in one batched transfer (cable_gpu_download_fields):
+        {name: float64 array}; out (optional) supplies the arrays to fill."""
+        out = {} if out is None else out
+        for k in names:
+            if k not in out:
+                out[k] = np.empty(BY_NAME[k].shape(self.mp), dtype=np.float64)
+        self._batched(lib().cable_gpu_download_fields, list(names), [out[k] for k in names])
+        return out
+
+    def _batched(self, fn, names, arrs):
+        n = len(names)
+        ids = np.array([BY_NAME[k].fid for k in names], dtype=np.int32)
+        ptrs = (ctypes.c_void_p * max(n, 1))(*[_ptr(a) for a in arrs])
+        types = np.full(n, HOST_F64, dtype=np.int32)
+        _check(fn(self.h, n, _ptr(ids), ptrs, _ptr(types)))
 
     def upload_all(self, fields: dict):
         for k, v in fields.items():

@@ -75,6 +75,21 @@ int cable_gpu_destroy(cable_gpu_ctx *ctx);
 int cable_gpu_upload(cable_gpu_ctx *ctx, int field_id, const void *host, int host_type);
 int cable_gpu_download(cable_gpu_ctx *ctx, int field_id, void *host, int host_type);
 
+/* Several fields in one transfer each way (what a drop-in moves around each
+   call): hosts[k] is the caller's array of field ids[k] in host type
+   host_types[k].  The library packs the fields into one pinned staging
+   buffer of the context (type conversion and tile order applied on up to
+   eight host threads), moves it with one copy and scatters it on the device
+   with one kernel; a download is the same in reverse.  Returns when the
+   host arrays (download) or the device fields (upload) hold the values.
+   Replaces the per-field transfers of the drop-in's host boundary
+   (cable_gpu_upload/download above); no counterpart in the reference, whose
+   cbm works on host memory (src/offline/cbl_model_driver_offline.F90:38). */
+int cable_gpu_upload_fields(cable_gpu_ctx *ctx, int n, const int *field_ids, const void *const *hosts,
+                            const int *host_types);
+int cable_gpu_download_fields(cable_gpu_ctx *ctx, int n, const int *field_ids, void *const *hosts,
+                              const int *host_types);
+
 /* Order of the tiles in device memory: device position j holds the caller's
    tile order[j] (a permutation of 0..mp-1).  Uploads, downloads and the
    resident forcing are permuted on the way in and out, so the caller keeps

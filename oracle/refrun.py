@@ -46,7 +46,7 @@ def _set(f, name, arr):
 
 
 def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
-        extra_sets=None, timeout=3600, poison=None, cpu=None, switches=None):
+        extra_sets=None, timeout=3600, poison=None, cpu=None, switches=None, env=None):
     """Run ``nsteps`` steps of the reference on workload ``wl``.
 
     outputs: field names to read back; with every_step they are read after
@@ -56,6 +56,7 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
     cpu: optional host core to pin the driver process to (taskset).
     switches: optional {switch: value} away from the shipped configuration
     (the driver's op 7), set before the first step.
+    env: optional extra environment variables of the driver process.
     Returns (dict, seconds the driver spent in its step operations -- cbm
     and the driver contract around it -- measured inside the driver process,
     without its file input and output).
@@ -92,7 +93,8 @@ def run(wl, nsteps=None, outputs=(), every_step=True, build="r8",
         # as the offline model is run with `ulimit -s unlimited`
         pin_cmd = ["taskset", "-c", str(cpu)] if cpu is not None else []
         subprocess.run(pin_cmd + [driver_path(build), pin, pout], check=True, timeout=timeout,
-                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, preexec_fn=_big_stack)
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, preexec_fn=_big_stack,
+                       env=None if env is None else {**os.environ, **env})
         dt = time.perf_counter() - t0
         res = {}
         with open(pout, "rb") as f:

@@ -24,15 +24,26 @@ def test_generated_bindings_are_current():
         assert fh.read() == gen_fields.generate(), "run python cable_amd/fortran/gen_fields.py"
 
 
+# balance diagnostics the offline driver's cable_checks computes after cbm,
+# and met%ofsd, which it sets before cbm; no file under the reference's
+# src/science touches them
+CHECKS = {"bal%wbal", "bal%wbal_tot", "bal%precip_tot", "bal%rnoff_tot", "bal%evap_tot", "bal%radbal",
+          "bal%ebal", "bal%ebalsoil", "bal%ebalveg", "bal%ebal_tot", "bal%radbalsum", "met%ofsd"}
+
+
 def test_bindings_cover_every_field():
+    """The drop-in moves every field cbm reads or writes, and none of the
+    driver's own: its state (drv%, sum_flux%) and the balance diagnostics
+    it computes itself (moving those would overwrite the host's running
+    totals with the device's stale copies)."""
     from cable_amd.fortran import gen_fields
     txt = gen_fields.generate()
     for f in FIELDS:
-        if f.name.startswith(("drv%", "sum_flux%")):
-            # the driver's own state: not an argument of cbm
+        if f.name.startswith(("drv%", "sum_flux%")) or f.name in CHECKS:
             assert f"C_LOC({f.name})" not in txt, f.name
             continue
         assert f"C_LOC({f.name})" in txt, f.name
+    assert all(n in {f.name for f in FIELDS} for n in CHECKS)
 
 
 def _flang():
@@ -105,6 +116,29 @@ def test_reference_driver_through_fortran_dropin_fp64():
                          ("canopy%fgpp", 1e-8), ("ssnow%tgg", 1e-5), ("ssnow%wb", 1e-7)):
             d = np.abs(out[k][step - 1] - exp["r8"][k][j]).max()
             assert d <= bound, (step, k, d)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refrun.available("gpu8"), reason="oracle/_ref/gpu8 not built")
+def test_resident_transfers_give_the_same_run():
+    """CABLE_GPU_SYNC=resident: after the first call the drop-in uploads only
+    what the offline driver writes between calls (forcing, oldcansto, vlai)
+    and the device keeps the rest -- the same bits as uploading every field
+    each call.  With a download list as well, the listed fields still come
+    back bit for bit (the host's copies of the others go stale, which only
+    the host's own balance diagnostics would see)."""
+    print("[fortran] gpu8 driver, resident transfers", flush=True)
+    case, _ = golden()
+    ref, _ = refrun.run(case, case.nsteps, REF_NAMES, every_step=True, build="gpu8", timeout=300)
+    res, _ = refrun.run(case, case.nsteps, REF_NAMES, every_step=True, build="gpu8", timeout=300,
+                        env={"CABLE_GPU_SYNC": "resident"})
+    for k in REF_NAMES:
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+    few = ["canopy%fe", "canopy%fh", "canopy%rnet", "canopy%fgpp", "ssnow%tgg", "ssnow%wb", "rad%trad"]
+    lst, _ = refrun.run(case, case.nsteps, few, every_step=True, build="gpu8", timeout=300,
+                        env={"CABLE_GPU_SYNC": "resident", "CABLE_GPU_DOWNLOAD": ",".join(few)})
+    for k in few:
+        np.testing.assert_array_equal(lst[k], ref[k], err_msg=k)
 
 
 @pytest.mark.gpu
