@@ -33,6 +33,37 @@ def test_oracle_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
 
+def test_oracle_matches_torch_forward_and_autograd():
+    """The oracle's forward pass and gradient (FP64, no BF16 rounding)
+    against torch's own Linear/SiLU layers and autograd on the CPU: an
+    implementation the oracle shares nothing with, so a transposed weight,
+    a wrong activation or a dropped bias term in either pass shows."""
+    import torch
+    rng = np.random.default_rng(4)
+    p = O.init_params(8)
+    x = rng.normal(size=(33, O.NIN))
+    t = rng.normal(size=(33, O.NOUT))
+    Ws, bs = O.unpack(p)
+    layers = []
+    for l, (W, b) in enumerate(zip(Ws, bs)):
+        lin = torch.nn.Linear(W.shape[0], W.shape[1]).double()
+        with torch.no_grad():
+            lin.weight.copy_(torch.from_numpy(W.T.copy()))
+            lin.bias.copy_(torch.from_numpy(b))
+        layers += [lin] + ([torch.nn.SiLU()] if l < 3 else [])
+    net = torch.nn.Sequential(*layers)
+    y = net(torch.from_numpy(x))
+    loss = torch.mean((y - torch.from_numpy(t)) ** 2)
+    loss.backward()
+    yo, _ = O.forward(p, x, bf16_ops=False)
+    lo, go = O.loss_and_grad(p, x, t, bf16_ops=False)
+    np.testing.assert_allclose(yo, y.detach().numpy(), rtol=1e-12, atol=1e-12)
+    assert abs(lo - loss.item()) <= 1e-12 * loss.item()
+    gt = np.concatenate([np.concatenate([m.weight.grad.numpy().T.reshape(-1), m.bias.grad.numpy()])
+                         for m in net if isinstance(m, torch.nn.Linear)])
+    np.testing.assert_allclose(go, gt, rtol=1e-10, atol=1e-14)
+
+
 def test_oracle_adam_matches_closed_form_first_step():
     # the first Adam step moves every parameter by lr * sign(g) (up to eps)
     g = np.array([0.3, -2.0, 1e-3])
