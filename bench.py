@@ -112,9 +112,6 @@ def class_costs(wl, prec, log):
 ORDER_LABEL = "grouped by vegetation type (measured most expensive first), then soil type and leaf area"
 
 
-ORDER_LABEL = "grouped by vegetation type (measured most expensive first), then soil type and leaf area"
-
-
 def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None):
     """Time ``steps`` resident steps of ``precision`` ("fp64", "mixed"; with
     ``hybrid``, the hybrid method) on the n-tile workload, after ``warmup``
