@@ -20,7 +20,7 @@ namespace cbm {
 
 namespace mlp = cbm_mlp;
 using cbm_mlp::f32_to_bf16;
-using cbm_mlp::silu;
+using cbm_mlp::hswish;
 using cbm_mlp::v16f;
 using cbm_mlp::v4s;
 
@@ -411,7 +411,7 @@ cbm_kernel(StepArgs a) {
 //
 // One workgroup per CU of eight waves steps blocks of 256 tiles, 32 per
 // wave, two waves per SIMD so that one wave's matrix-core products overlap
-// the other's SiLU.  The weights (BF16, transposed [out][in], rows padded by
+// the other's activations.  The weights (BF16, transposed [out][in], rows padded by
 // 8) and biases are staged once per workgroup into LDS; each wave keeps its 32 activation rows
 // in LDS between layers.  A layer is computed transposed, D = W^T x A^T on
 // v_mfma_f32_32x32x16_bf16: a lane then holds four consecutive outputs of
@@ -433,7 +433,6 @@ static_assert(2 * (WLDS + ALDS) + 4 * BLDS <= 160 * 1024, "fits the 160 KB of LD
 constexpr int G1 = 0, G2 = G1 + HID * NIN, G3 = G2 + HID * HID, G4 = G3 + HID * HID, GTOT = G4 + NOUT * HID;
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
-__device__ inline float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 }  // namespace infer
 
 // (a template so that the header's kernel has one definition across the
@@ -568,7 +567,7 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
             const int j = nb * 32 + 8 * g + kq;
             bf4 v;
 #pragma unroll
-            for (int q = 0; q < 4; q++) v[q] = (__bf16)silu_fast(acc[nb][4 * g + q] + b[j + q]);
+            for (int q = 0; q < 4; q++) v[q] = (__bf16)hswish(acc[nb][4 * g + q] + b[j + q]);
             *(bf4 *)&A[col * AP + j] = v;
           }
         __builtin_amdgcn_wave_barrier();

@@ -117,9 +117,9 @@ struct Tile {
 enum StepMode {
   MODE_PLAIN = 0,    // the reference's method
   MODE_CAPTURE = 1,  // the reference's method, and write surrogate training samples
-  MODE_HYBRID = 2    // surrogate warm start, capped polish passes judged by
+  MODE_HYBRID = 2    // surrogate warm start, capped polish passes checked by
                      // the leaf energy residual, the reference's iteration
-                     // where the judge rejects them
+                     // where the check rejects them
 };
 
 // globals of one step

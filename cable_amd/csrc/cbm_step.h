@@ -26,7 +26,7 @@ struct Stepper {
   bool fell_back = false;
   bool vegetated = false;
   bool capped = false;         // MODE_HYBRID: a sunlit leaf-bearing tile, whose
-                               // leaf passes are capped and judged
+                               // leaf passes are capped and checked
   R leaf_temperature = R(0);   // the leaf fixed point's final temperature
 
   // ---- constants: src/params/*, src/util/cable_common.F90:217-222 ----
@@ -593,7 +593,7 @@ struct Stepper {
     const bool leaves = fvl0 + fvl1 > LAI_THRESH;
     // the reference's fixed count of 20 passes.  The hybrid polishes the
     // start of a sunlit leaf-bearing tile with a.max_passes passes and then
-    // judges it by its leaf energy-balance residual: the leaf
+    // checks it by its leaf energy-balance residual: the leaf
     // sensible-plus-radiative conductance times the temperature change the
     // last pass still made, beyond the 0.1 K the reference's own convergence
     // test accepts, cr (ghr1 + ghr2) (|dT| - 0.1 K).  Within a.resid_max

@@ -1,5 +1,5 @@
 // mlp.hip -- the canopy surrogate: a 24 -> 128 -> 128 -> 128 -> 8 MLP with
-// SiLU, trained on the GPU: BF16 operands on the matrix cores
+// hard-swish, trained on the GPU: BF16 operands on the matrix cores
 // (v_mfma_f32_32x32x16_bf16) with FP32 accumulation, FP32 master weights and
 // Adam.  All kernels are hand-written for gfx950 wave64.
 //
@@ -9,7 +9,7 @@
 // It gathers and normalises their samples, runs the forward pass and the
 // backward pass down to the first layer's pre-activation gradient, and forms
 // its rows' share of every parameter's gradient.  Wave w owns features
-// 32w .. 32w+31 of every 128-wide layer in both passes, so the SiLU
+// 32w .. 32w+31 of every 128-wide layer in both passes, so the activation's
 // derivative it needs on the way back is the one it computed on the way
 // forward, still in its registers; activations and gradients pass between
 // layers through LDS (one workgroup barrier per layer).  Each wave loads its
@@ -140,8 +140,6 @@ struct UpdateArgs {
   float lr, b1, b2, eps, c1, c2;
 };
 
-__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-
 // the sum, over the 32 lanes of a half-wave, of each of a lane's NV values.
 // Each exchange step halves the values a lane carries: the lane keeps one
 // half, adds its partner's copy of it and passes on the other half; after
@@ -210,9 +208,9 @@ __device__ __forceinline__ v16f product(const bf8 *W, const __bf16 *in, int col,
   return a0;
 }
 
-// a hidden layer forward: z = XW + b, h = silu(z) into `out` (and into the
+// a hidden layer forward: z = XW + b, h = hswish(z) into `out` (and into the
 // block's feature-major H rows `ht` when the backward pass follows), with
-// silu'(z) kept in sg
+// hswish'(z) kept in sg
 template <int KS>
 __device__ __forceinline__ void forward_layer(const bf8 *W, const __bf16 *in, __bf16 *out, const float *b,
                                               float *sg, __bf16 *ht, int wave, int col, int kh, int kq) {
@@ -224,9 +222,8 @@ __device__ __forceinline__ void forward_layer(const bf8 *W, const __bf16 *in, __
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const float z = acc[4 * g + q] + b[j + q];
-      const float s = sigmoid(z);
-      sg[4 * g + q] = s * (1.0f + z * (1.0f - s));
-      hv[q] = (__bf16)(z * s);
+      sg[4 * g + q] = hswish_grad(z);
+      hv[q] = (__bf16)hswish(z);
     }
     *(bf4 *)&out[col * AP + j] = hv;
     if (ht)
@@ -235,7 +232,7 @@ __device__ __forceinline__ void forward_layer(const bf8 *W, const __bf16 *in, __
   }
 }
 
-// a hidden layer backward: dZ = (dZ_next W^T) silu'(z), into `out` (when a
+// a hidden layer backward: dZ = (dZ_next W^T) hswish'(z), into `out` (when a
 // product follows) and the block's feature-major dZ rows `dt`; the bias
 // gradient's FP32 share from this block into the partials at bias0
 template <int KS>

@@ -63,7 +63,7 @@ def _f32(a, shape=None):
 
 
 def init_params(seed=0):
-    """He-uniform weights (SiLU layers), zero biases, a half-scale output layer."""
+    """He-uniform weights (hard-swish layers), zero biases, a half-scale output layer."""
     rng = np.random.default_rng(seed)
     parts = []
     for l, (i, o) in enumerate(SHAPES):

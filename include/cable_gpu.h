@@ -290,7 +290,7 @@ int cable_gpu_set_surrogate(cable_gpu_ctx *ctx, const float *params, const float
 /* Run steps k0..k0+nsteps-1 with the hybrid method: per step the network
    predicts the leaf state of each sunlit leaf-bearing tile and the leaf
    fixed point of those tiles starts there.  After max_passes polish passes
-   (per NITER pass) such a tile is judged by its leaf energy-balance
+   (per NITER pass) such a tile is checked by its leaf energy-balance
    residual beyond the reference's 0.1 K tolerance, cr (ghr1 + ghr2)
    (|dT| - 0.1 K): within resid_max (W/m2) it stops; otherwise it falls back
    to the reference's iteration, which goes on with the reference's damping

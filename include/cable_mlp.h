@@ -1,9 +1,10 @@
 /*
  * cable_mlp.h -- C ABI of the canopy surrogate in libcable_gpu.so.
  *
- * A 24 -> 128 -> 128 -> 128 -> 8 multilayer perceptron with SiLU
- * activations, trained on the GPU by hand-written kernels: BF16 operands on
- * the matrix cores with FP32 accumulation, FP32 master weights, Adam
+ * A 24 -> 128 -> 128 -> 128 -> 8 multilayer perceptron with hard-swish
+ * activations (z min(max(z + 3, 0), 6) / 6), trained on the GPU by
+ * hand-written kernels: BF16 operands on the matrix cores with FP32
+ * accumulation, FP32 master weights, Adam
  * (beta1 0.9, beta2 0.999, eps 1e-8), mean-squared error on normalised
  * outputs.  It maps a tile's forcing, state and parameters at the start of a
  * step to the converged state of the reference's leaf fixed point

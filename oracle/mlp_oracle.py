@@ -3,7 +3,7 @@
 Test infrastructure only: tests/ use it to check the GPU forward pass, the
 backward pass and Adam.  It restates the network in numpy:
 
-    z1 = x.W1 + b1, h1 = silu(z1), ..., y = h3.W4 + b4     (24-128-128-128-8)
+    z1 = x.W1 + b1, h1 = hswish(z1), ..., y = h3.W4 + b4     (24-128-128-128-8)
     loss = mean((y - t)^2)
 
 and its gradient by the chain rule.  With ``bf16=True`` every operand of a
@@ -45,13 +45,13 @@ def pack(Ws, bs):
     return np.concatenate([np.concatenate([W.reshape(-1), b]) for W, b in zip(Ws, bs)])
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def hswish(z):
+    """Hard-swish, z * min(max(z + 3, 0), 6) / 6 (Howard et al., MobileNetV3)."""
+    return z * np.clip(z + 3.0, 0.0, 6.0) / 6.0
 
 
-def silu_grad(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def hswish_grad(z):
+    return np.where(z < -3.0, 0.0, np.where(z > 3.0, 1.0, (2.0 * z + 3.0) / 6.0))
 
 
 def _mm(a, b, use_bf16):
@@ -68,7 +68,7 @@ def forward(params, x, bf16_ops=True):
         z = _mm(h, Ws[l], bf16_ops) + bs[l]
         if l < 3:
             zs.append(z)
-            h = silu(z)
+            h = hswish(z)
             hs.append(h)
         else:
             h = z
@@ -89,7 +89,7 @@ def loss_and_grad(params, x, t, bf16_ops=True):
         gb[l] = dcur.sum(axis=0)
         if l == 0:
             break
-        dcur = _mm(dcur, Ws[l].T, bf16_ops) * silu_grad(zs[l - 1])
+        dcur = _mm(dcur, Ws[l].T, bf16_ops) * hswish_grad(zs[l - 1])
     return loss, pack(gW, gb)
 
 
@@ -102,7 +102,7 @@ def adam(params, grads, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
 
 
 def init_params(seed=0):
-    """He-uniform weights for SiLU layers, zero biases, small output layer."""
+    """He-uniform weights for the hard-swish layers, zero biases, small output layer."""
     rng = np.random.default_rng(seed)
     Ws, bs = [], []
     for l, (i, o) in enumerate(SHAPES):

@@ -35,7 +35,7 @@ def test_oracle_gradient_matches_finite_differences():
 
 def test_oracle_matches_torch_forward_and_autograd():
     """The oracle's forward pass and gradient (FP64, no BF16 rounding)
-    against torch's own Linear/SiLU layers and autograd on the CPU: an
+    against torch's own Linear/Hardswish layers and autograd on the CPU: an
     implementation the oracle shares nothing with, so a transposed weight,
     a wrong activation or a dropped bias term in either pass shows."""
     import torch
@@ -50,7 +50,7 @@ def test_oracle_matches_torch_forward_and_autograd():
         with torch.no_grad():
             lin.weight.copy_(torch.from_numpy(W.T.copy()))
             lin.bias.copy_(torch.from_numpy(b))
-        layers += [lin] + ([torch.nn.SiLU()] if l < 3 else [])
+        layers += [lin] + ([torch.nn.Hardswish()] if l < 3 else [])
     net = torch.nn.Sequential(*layers)
     y = net(torch.from_numpy(x))
     loss = torch.mean((y - torch.from_numpy(t)) ** 2)
