@@ -109,7 +109,8 @@ def class_costs(wl, prec, log):
     return _COSTS[prec]
 
 
-ORDER_LABEL = "grouped by vegetation type (measured most expensive first), then soil type and leaf area"
+ORDER_LABEL = ("grouped by vegetation type (measured most expensive first), then 4 K band of the tile's mean air "
+               "temperature, soil type and leaf area")
 
 
 def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None):
@@ -123,7 +124,8 @@ def run_form(n, steps, warmup, precision, log, class_order=True, hybrid=None):
     wl = Workload(n, nsteps)
     log(f"[bench] workload {n} tiles x {nsteps} steps generated in {time.perf_counter() - t:.1f}s")
     prec = gpu.PREC_FP64 if precision == "fp64" else gpu.PREC_MIXED
-    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(wl, prec, log)) if class_order else None
+    order = (gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(wl, prec, log), tair=wl.t0) if class_order
+             else None)
 
     def ctx():
         g = gpu.from_workload(wl, precision=prec, full=False, order=order)
@@ -180,7 +182,7 @@ def gate_stats(n, steps, params, norm, log):
     from cable_amd.workload import Workload
     t = time.perf_counter()
     wl = Workload(n, steps)
-    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai)
+    order = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, tair=wl.t0)
     ctx = {"fp64": gpu.from_workload(wl, precision=gpu.PREC_FP64, full=False, order=order),
            "mixed": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order),
            "hybrid": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=order)}
@@ -369,8 +371,8 @@ def production(args, log):
     log(f"[bench] production workload {n} tiles generated in {time.perf_counter() - t:.1f}s")
     params, norm, info = surrogate.train_on_gpu(log=log, **TRAIN)
     cal = Workload(65536, 8)        # calibration of the longest-first order
-    o64 = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(cal, gpu.PREC_FP64, log))
-    o32 = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(cal, gpu.PREC_MIXED, log))
+    o64 = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(cal, gpu.PREC_FP64, log), tair=wl.t0)
+    o32 = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, class_costs(cal, gpu.PREC_MIXED, log), tair=wl.t0)
     ctx = {"fp64": gpu.from_workload(wl, precision=gpu.PREC_FP64, full=False, order=o64),
            "mixed": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=o32),
            "hybrid": gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=o32)}

@@ -192,7 +192,7 @@ def capture_samples(n_tiles=262144, nsteps=64, seed=None, order=True):
     from . import gpu
     from .workload import DEFAULT_SEED, Workload
     wl = Workload(n_tiles, nsteps, seed=(DEFAULT_SEED + 1) if seed is None else seed)
-    o = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai) if order else None
+    o = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, tair=wl.t0) if order else None
     g = gpu.from_workload(wl, precision=gpu.PREC_MIXED, full=False, order=o)
     n = g.run_capture(0, nsteps, wl.dels, n_tiles * nsteps)
     return g, n

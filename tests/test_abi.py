@@ -118,3 +118,12 @@ def test_class_order_is_a_permutation_grouped_by_type():
     v = wl.iveg[gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, cost)]
     runs = v[np.r_[True, v[1:] != v[:-1]]].tolist()
     assert runs == sorted(runs, key=lambda c: (-cost[c], c))
+    # with a characteristic air temperature, a type's tiles go in 4 K bands
+    # of it, then soil type, then leaf area
+    o = gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, cost, tair=wl.t0)
+    assert sorted(o.tolist()) == list(range(wl.n))
+    v, band, soil, lai = wl.iveg[o], np.floor(wl.t0[o] / 4.0), wl.isoilm[o], wl.vlai[o]
+    for c in np.unique(v):
+        m = v == c
+        key = list(zip(band[m], soil[m], lai[m]))
+        assert key == sorted(key)

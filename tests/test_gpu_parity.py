@@ -178,7 +178,7 @@ def test_tile_order_is_invisible(precision):
     a, pa = _trajectories(wl, precision, names, full=False)
     cost = {c: float((7 * c) % 5) for c in range(1, 18)}     # any ranking of the types
     b, pb = _trajectories(wl, precision, names, full=False,
-                          order=gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, cost))
+                          order=gpu.class_order(wl.iveg, wl.isoilm, wl.vlai, cost, tair=wl.t0))
     diff = [k for k in names if not np.array_equal(a[k], b[k])]
     assert not diff, f"fields differing: {diff}"
     assert pa == pb
