@@ -80,3 +80,25 @@ def test_every_field_round_trips_in_one_batch():
     g.close()
     for n in names:
         np.testing.assert_array_equal(back[n], vals[n], err_msg=n)
+
+
+def test_batched_transfers_refuse_bad_arguments():
+    """A field id out of range or an unknown host type is refused with a
+    message, before anything moves; an empty batch is a no-op."""
+    import ctypes
+    from cable_amd import gpu
+    print("[transfers] bad arguments", flush=True)
+    g = gpu.CableGPU(64, 0)
+    L = gpu.lib()
+    a = np.zeros(64)
+    ptrs = (ctypes.c_void_p * 1)(a.ctypes.data)
+    for fid, ht in ((10000, gpu.HOST_F64), (-1, gpu.HOST_F64), (BY_NAME["canopy%fe"].fid, 7)):
+        ids = np.array([fid], dtype=np.int32)
+        types = np.array([ht], dtype=np.int32)
+        for fn in (L.cable_gpu_upload_fields, L.cable_gpu_download_fields):
+            assert fn(g.h, 1, ids.ctypes.data, ptrs, types.ctypes.data) != 0
+            assert b"fields" in L.cable_gpu_last_error()
+    g.upload_fields({})
+    assert g.download_fields([]) == {}
+    g.close()
+
