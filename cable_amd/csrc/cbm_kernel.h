@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
   static_assert(NWAVES == WAVES, "LDS sized for WAVES");
   __shared__ __attribute__((aligned(16))) __bf16 wl[WLDS];
   __shared__ __attribute__((aligned(16))) __bf16 act[ALDS];
-  __shared__ float bl[BLDS];
+  __shared__ __attribute__((aligned(16))) float bl[BLDS];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31;
   const int kh = 8 * (lane >> 5), kq = 4 * (lane >> 5);
   __bf16 *A = &act[wave * 32 * AP];
@@ -544,11 +544,20 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
         const int K = l == 0 ? K1 : HID;
         const __bf16 *W = &wl[l == 0 ? OFF1 : (l == 1 ? OFF2 : OFF3)];
         const int pw = l == 0 ? P1 : PH;
+        // lane: tile `col`; register group g of block nb: outputs nb*32 + 8g + kq + 0..3.
+        // The accumulators start at the bias, so the products add onto it.
+        const float *b = &bl[l * HID];
         v16f acc[4];
 #pragma unroll
         for (int nb = 0; nb < 4; nb++)
 #pragma unroll
-          for (int r = 0; r < 16; r++) acc[nb][r] = 0.0f;
+          for (int g = 0; g < 4; g++) {
+            const float4 bv = *(const float4 *)&b[nb * 32 + 8 * g + kq];
+            acc[nb][4 * g + 0] = bv.x;
+            acc[nb][4 * g + 1] = bv.y;
+            acc[nb][4 * g + 2] = bv.z;
+            acc[nb][4 * g + 3] = bv.w;
+          }
         for (int k0 = 0; k0 < K; k0 += 16) {
           const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
 #pragma unroll
@@ -558,8 +567,6 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
           }
         }
         __builtin_amdgcn_wave_barrier();
-        // lane: tile `col`; register group g of block nb: outputs nb*32 + 8g + kq + 0..3
-        const float *b = &bl[l * HID];
 #pragma unroll
         for (int nb = 0; nb < 4; nb++)
 #pragma unroll
@@ -567,7 +574,7 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
             const int j = nb * 32 + 8 * g + kq;
             bf4 v;
 #pragma unroll
-            for (int q = 0; q < 4; q++) v[q] = (__bf16)hswish(acc[nb][4 * g + q] + b[j + q]);
+            for (int q = 0; q < 4; q++) v[q] = (__bf16)hswish(acc[nb][4 * g + q]);
             *(bf4 *)&A[col * AP + j] = v;
           }
         __builtin_amdgcn_wave_barrier();

@@ -83,7 +83,12 @@ __host__ __device__ inline int64_t shuffled_sample(uint64_t seed, int64_t pos, i
 // MobileNetV3", ICCV 2019).  It costs four plain VALU instructions where
 // SiLU costs an exponential and a reciprocal, which took half the inference
 // kernel's VALU cycles.  h'(z) = 0 below -3, (2z + 3) / 6 between, 1 above.
-__host__ __device__ inline float hswish(float z) { return z * fminf(fmaxf(z + 3.0f, 0.0f), 6.0f) * (1.0f / 6.0f); }
+// Written as z clamp(z/6 + 1/2, 0, 1), the same function: the clamp to [0, 1]
+// folds into the fused multiply-add's output modifier, so h is two VALU
+// instructions (the form z min(max(z + 3, 0), 6) / 6 takes four).
+__host__ __device__ inline float hswish(float z) {
+  return z * fminf(fmaxf(fmaf(z, 1.0f / 6.0f, 0.5f), 0.0f), 1.0f);
+}
 __host__ __device__ inline float hswish_grad(float z) {
   return z < -3.0f ? 0.0f : z > 3.0f ? 1.0f : (2.0f * z + 3.0f) * (1.0f / 6.0f);
 }
