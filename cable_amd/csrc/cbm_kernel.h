@@ -21,6 +21,7 @@ namespace cbm {
 namespace mlp = cbm_mlp;
 using cbm_mlp::f32_to_bf16;
 using cbm_mlp::hswish;
+using cbm_mlp::lds_barrier;
 using cbm_mlp::v16f;
 using cbm_mlp::v4s;
 
@@ -56,22 +57,73 @@ constexpr int role_id(int r) { return r; }
 // surrogate inputs: 24 features of a tile at the start of a step, read from
 // the device arrays (the resident run's layout: forcing at a.foff) and the
 // class-table rows.  The capture step and the inference kernel both call
-// this, so training and use see the same features.
+// this, so training and use see the same features.  The loads and the
+// arithmetic are separate steps, so that the inference can issue one block's
+// loads before the previous block's network and finish them after it.
 template <class R>
-CBM_HD void surrogate_features(const StepArgs &a, int64_t t, const double *vrow, const double *srow, float *x) {
+struct FeatureRaw {
   using DT = typename DevT<R, 'F'>::type;
+  DT tk, qv, fsd, fld, ua, pmb, coszen, vlai, tss, snowd, cansto, rtsoil;
+  double wb[MS], wbice[MS];
+  float hyb[3];
+  int iveg, isoilm;
+};
+
+// the device arrays the features read, looked up once: a kernel that stores
+// between feature loads would otherwise reload every pointer from a.F
+struct FeatureSrc {
+  const void *tk, *qv, *fsd, *fld, *ua, *pmb, *coszen, *iveg, *isoilm, *vlai, *wb, *wbice, *tss, *snowd, *cansto,
+      *rtsoil;
+};
+CBM_HD inline FeatureSrc feature_src(const StepArgs &a) {
   void *const *F = a.F;
+  return {F[CABLE_F_met_tk],       F[CABLE_F_met_qv],      F[CABLE_F_met_fsd],   F[CABLE_F_met_fld],
+          F[CABLE_F_met_ua],       F[CABLE_F_met_pmb],     F[CABLE_F_met_coszen], F[CABLE_F_veg_iveg],
+          F[CABLE_F_soil_isoilm],  F[CABLE_F_veg_vlai],    F[CABLE_F_ssnow_wb],  F[CABLE_F_ssnow_wbice],
+          F[CABLE_F_ssnow_tss],    F[CABLE_F_ssnow_snowd], F[CABLE_F_canopy_cansto], F[CABLE_F_ssnow_rtsoil]};
+}
+
+template <class R>
+CBM_HD void load_feature_raw(const StepArgs &a, const FeatureSrc &p, int64_t t, FeatureRaw<R> &r) {
+  using DT = typename DevT<R, 'F'>::type;
   const int64_t mp = a.mp, f = t + a.foff;
-  const float tk = (float)ld<DT>(F[CABLE_F_met_tk], f), qv = (float)ld<DT>(F[CABLE_F_met_qv], f);
-  const int iveg = ld<int>(F[CABLE_F_veg_iveg], t);
-  const float vlai = iveg >= 14 ? 0.0f : (float)ld<DT>(F[CABLE_F_veg_vlai], t);
-  x[0] = (float)ld<DT>(F[CABLE_F_met_fsd], f);
-  x[1] = (float)ld<DT>(F[CABLE_F_met_fld], f);
+  r.tk = ld<DT>(p.tk, f);
+  r.qv = ld<DT>(p.qv, f);
+  r.fsd = ld<DT>(p.fsd, f);
+  r.fld = ld<DT>(p.fld, f);
+  r.ua = ld<DT>(p.ua, f);
+  r.pmb = ld<DT>(p.pmb, f);
+  r.coszen = ld<DT>(p.coszen, f);
+  r.iveg = ld<int>(p.iveg, t);
+  r.isoilm = ld<int>(p.isoilm, t);
+  r.vlai = ld<DT>(p.vlai, t);
+#pragma unroll
+  for (int k = 0; k < MS; k++) {
+    r.wb[k] = ld<double>(p.wb, k * mp + t);
+    r.wbice[k] = ld<double>(p.wbice, k * mp + t);
+  }
+  const bool h = a.hyb_valid && a.hyb;
+#pragma unroll
+  for (int c = 0; c < 3; c++) r.hyb[c] = h ? a.hyb[c * mp + t] : 0.0f;
+  r.tss = ld<DT>(p.tss, t);
+  r.snowd = ld<DT>(p.snowd, t);
+  r.cansto = ld<DT>(p.cansto, t);
+  r.rtsoil = ld<DT>(p.rtsoil, t);
+}
+
+// vrow, srow: the rows of the tile's vegetation type and soil type
+template <class R>
+CBM_HD void finish_features(const StepArgs &a, const FeatureRaw<R> &r, const double *vrow, const double *srow,
+                            float *x) {
+  const float tk = (float)r.tk, qv = (float)r.qv;
+  const float vlai = r.iveg >= 14 ? 0.0f : (float)r.vlai;
+  x[0] = (float)r.fsd;
+  x[1] = (float)r.fld;
   x[2] = tk - 273.16f;
   x[3] = qv * 1.0e3f;
-  x[4] = (float)ld<DT>(F[CABLE_F_met_ua], f);
-  x[5] = (float)ld<DT>(F[CABLE_F_met_pmb], f);
-  x[6] = (float)ld<DT>(F[CABLE_F_met_coszen], f);
+  x[4] = (float)r.ua;
+  x[5] = (float)r.pmb;
+  x[6] = (float)r.coszen;
   x[7] = vlai;
   x[8] = (float)vrow[VC_vcmax] * 1.0e6f;
   x[9] = (float)vrow[VC_frac4];
@@ -85,22 +137,29 @@ CBM_HD void surrogate_features(const StepArgs &a, int64_t t, const double *vrow,
   float s = 0.0f;
 #pragma unroll
   for (int k = 0; k < MS; k++) {
-    const double wl = ld<double>(F[CABLE_F_ssnow_wb], k * mp + t) - ld<double>(F[CABLE_F_ssnow_wbice], k * mp + t);
+    const double wl = r.wb[k] - r.wbice[k];
     const double sw = srow[SC_swilt_vec + k], fc = srow[SC_sfc_vec + k];
-    float r = (float)((wl - sw) / (fc - sw));
-    r = r < 1.0e-9f ? 1.0e-9f : (r > 1.0f ? 1.0f : r);
-    s += (float)vrow[VC_froot + k] * r;
+    float q = (float)((wl - sw) / (fc - sw));
+    q = q < 1.0e-9f ? 1.0e-9f : (q > 1.0f ? 1.0f : q);
+    s += (float)vrow[VC_froot + k] * q;
   }
   float fw = (float)vrow[VC_vbeta] * (s > 1.0e-9f ? s : 1.0e-9f);
   x[16] = fw < 1.0e-9f ? 1.0e-9f : (fw > 1.0f ? 1.0f : fw);
   const bool h = a.hyb_valid && a.hyb;
-  x[17] = (h ? a.hyb[t] : tk) - tk;
-  x[18] = (h ? a.hyb[mp + t] : tk) - tk;
-  x[19] = ((h ? a.hyb[2 * mp + t] : qv) - qv) * 1.0e3f;
-  x[20] = (float)ld<DT>(F[CABLE_F_ssnow_tss], t) - tk;
-  x[21] = (float)ld<DT>(F[CABLE_F_ssnow_snowd], t);
-  x[22] = (float)ld<DT>(F[CABLE_F_canopy_cansto], t);
-  x[23] = (float)ld<DT>(F[CABLE_F_ssnow_rtsoil], t);
+  x[17] = (h ? r.hyb[0] : tk) - tk;
+  x[18] = (h ? r.hyb[1] : tk) - tk;
+  x[19] = ((h ? r.hyb[2] : qv) - qv) * 1.0e3f;
+  x[20] = (float)r.tss - tk;
+  x[21] = (float)r.snowd;
+  x[22] = (float)r.cansto;
+  x[23] = (float)r.rtsoil;
+}
+
+template <class R>
+CBM_HD void surrogate_features(const StepArgs &a, int64_t t, const double *vrow, const double *srow, float *x) {
+  FeatureRaw<R> r;
+  load_feature_raw<R>(a, feature_src(a), t, r);
+  finish_features<R>(a, r, vrow, srow, x);
 }
 
 // wave-wide append: lanes with `take` get consecutive slots from *count
@@ -428,7 +487,9 @@ constexpr int OFF1 = 0, OFF2 = OFF1 + HID * P1, OFF3 = OFF2 + HID * PH, OFF4 = O
 constexpr int WLDS = OFF4 + NOUT * PH;           // 41,024 BF16
 constexpr int ALDS = WAVES * 32 * AP;            // 34,816 BF16
 constexpr int BLDS = 3 * HID + NOUT;             // biases, FP32
-static_assert(2 * (WLDS + ALDS) + 4 * BLDS <= 160 * 1024, "fits the 160 KB of LDS");
+static_assert(2 * (WLDS + ALDS) + 4 * BLDS + 8 * (NVEGCLASS * VC_N + NSOILCLASS * SC_N) + 4 * (VCHUNK + NNORM) <=
+                  160 * 1024,
+              "fits the 160 KB of LDS");
 // the transposed global layout: per layer [out][in], layers back to back
 constexpr int G1 = 0, G2 = G1 + HID * NIN, G3 = G2 + HID * HID, G4 = G3 + HID * HID, GTOT = G4 + NOUT * HID;
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -458,26 +519,50 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
   // RAD_THRESH in FP32).  The workgroup takes the votes of VCHUNK of its
   // blocks at once (loads issued together), then steps the blocks with work.
   __shared__ int vote[VCHUNK];
+  // the class tables (the features' table columns) and the outputs'
+  // normalisation, so that no global load but the features' own is waited
+  // for between a block's feature loads and its network
+  __shared__ double tv[NVEGCLASS * VC_N], ts[NSOILCLASS * SC_N];
+  __shared__ float nrm[NNORM];
+  for (int i = threadIdx.x; i < NVEGCLASS * VC_N; i += blockDim.x) tv[i] = a.vtab[i];
+  for (int i = threadIdx.x; i < NSOILCLASS * SC_N; i += blockDim.x) ts[i] = a.stab[i];
+  if (threadIdx.x < NNORM) nrm[threadIdx.x] = a.norm[threadIdx.x];
+  const FeatureSrc src = feature_src(a);
   const int64_t G = gridDim.x;
+  const bool feat_lane = threadIdx.x < TILES;
   for (int64_t c0 = blockIdx.x; c0 < nblk; c0 += G * VCHUNK) {
     __syncthreads();  // the previous chunk's blocks are done with vote[]
     if (threadIdx.x < VCHUNK) vote[threadIdx.x] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < VCHUNK * TILES; i += blockDim.x) {
       const int64_t b = c0 + (int64_t)(i / TILES) * G, t = b * TILES + i % TILES;
-      if (b < nblk && t < a.mp && ld<int>(a.F[CABLE_F_veg_iveg], t) < 14) {
-        const float fsd = ld<float>(a.F[CABLE_F_met_fsd], t + a.foff);
+      if (b < nblk && t < a.mp && ld<int>(src.iveg, t) < 14) {
+        const float fsd = ld<float>(src.fsd, t + a.foff);
         if (fsd + fsd > 0.001f) vote[i / TILES] = 1;
       }
     }
     __syncthreads();
-    for (int qb = 0; qb < VCHUNK; qb++) {
-      const int64_t blk = c0 + (int64_t)qb * G;
-      if (blk >= nblk) break;
-      if (!vote[qb] && !xraw) continue;
-      const int64_t t0 = blk * TILES;
+    // the chunk's first block at or after qb with work (VCHUNK: none); the
+    // same for every thread
+    auto next_block = [&](int qb) {
+      for (; qb < VCHUNK; qb++) {
+        if (c0 + (int64_t)qb * G >= nblk) return VCHUNK;
+        if (vote[qb] || xraw) return qb;
+      }
+      return VCHUNK;
+    };
+    // a block's feature loads are issued before the previous block's network
+    // and finished after it, so their latency hides behind the matrix work
+    FeatureRaw<float> raw;
+    int qb = next_block(0);
+    if (qb < VCHUNK && feat_lane) {
+      const int64_t t = (c0 + (int64_t)qb * G) * TILES + threadIdx.x;
+      if (t < a.mp) load_feature_raw<float>(a, src, t, raw);
+    }
+    while (qb < VCHUNK) {
+      const int64_t t0 = (c0 + (int64_t)qb * G) * TILES;
       // the previous block's layers are done with the activation rows
-      __syncthreads();
+      lds_barrier();
       // the weights are staged in LDS once per workgroup (the grid is one
       // workgroup per CU, each stepping every gridDim.x-th block of 256
       // tiles), at its first block with work, so a night step reads none; the
@@ -511,19 +596,19 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
         staged = true;
       }
       // features of tile threadIdx.x, normalised, as BF16 row threadIdx.x
-      if (threadIdx.x < TILES) {
+      if (feat_lane) {
         const int64_t t = t0 + threadIdx.x;
         __bf16 *row = &act[threadIdx.x * AP];
         float x[NIN];
         if (t < a.mp) {
-          const int iv = ld<int>(a.F[CABLE_F_veg_iveg], t) - 1, is = ld<int>(a.F[CABLE_F_soil_isoilm], t) - 1;
-          surrogate_features<float>(a, t, a.vtab + (iv < 0 ? 0 : iv) * VC_N, a.stab + (is < 0 ? 0 : is) * SC_N, x);
+          const int iv = raw.iveg - 1, is = raw.isoilm - 1;
+          finish_features<float>(a, raw, &tv[(iv < 0 ? 0 : iv) * VC_N], &ts[(is < 0 ? 0 : is) * SC_N], x);
           if (xraw) {
 #pragma unroll
             for (int c = 0; c < NIN; c++) xraw[t * NIN + c] = x[c];
           }
 #pragma unroll
-          for (int c = 0; c < NIN; c++) x[c] = (x[c] - a.norm[c]) / a.norm[NIN + c];
+          for (int c = 0; c < NIN; c++) x[c] = (x[c] - nrm[c]) / nrm[NIN + c];
         } else {
 #pragma unroll
           for (int c = 0; c < NIN; c++) x[c] = 0.0f;
@@ -536,7 +621,13 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
         *(uint2 *)&row[NIN] = make_uint2(0u, 0u);
         *(uint2 *)&row[NIN + 4] = make_uint2(0u, 0u);
       }
-      __syncthreads();
+      // (LDS only: the staged weights and the feature rows)
+      lds_barrier();
+      const int qn = next_block(qb + 1);
+      if (qn < VCHUNK && feat_lane) {
+        const int64_t t = (c0 + (int64_t)qn * G) * TILES + threadIdx.x;
+        if (t < a.mp) load_feature_raw<float>(a, src, t, raw);
+      }
       // from here on a wave touches only its own 32 activation rows: its LDS
       // operations complete in order, so the layers need no workgroup barrier
 #pragma unroll 1
@@ -597,13 +688,14 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
         const int64_t t = t0 + wave * 32 + col;
         if (t < a.mp) {
           float4 o;
-          o.x = (acc[0] + bl[3 * HID + kq + 0]) * a.norm[2 * NIN + NOUT + kq + 0] + a.norm[2 * NIN + kq + 0];
-          o.y = (acc[1] + bl[3 * HID + kq + 1]) * a.norm[2 * NIN + NOUT + kq + 1] + a.norm[2 * NIN + kq + 1];
-          o.z = (acc[2] + bl[3 * HID + kq + 2]) * a.norm[2 * NIN + NOUT + kq + 2] + a.norm[2 * NIN + kq + 2];
-          o.w = (acc[3] + bl[3 * HID + kq + 3]) * a.norm[2 * NIN + NOUT + kq + 3] + a.norm[2 * NIN + kq + 3];
+          o.x = (acc[0] + bl[3 * HID + kq + 0]) * nrm[2 * NIN + NOUT + kq + 0] + nrm[2 * NIN + kq + 0];
+          o.y = (acc[1] + bl[3 * HID + kq + 1]) * nrm[2 * NIN + NOUT + kq + 1] + nrm[2 * NIN + kq + 1];
+          o.z = (acc[2] + bl[3 * HID + kq + 2]) * nrm[2 * NIN + NOUT + kq + 2] + nrm[2 * NIN + kq + 2];
+          o.w = (acc[3] + bl[3 * HID + kq + 3]) * nrm[2 * NIN + NOUT + kq + 3] + nrm[2 * NIN + kq + 3];
           *(float4 *)&pred[t * NOUT + kq] = o;
         }
       }
+      qb = qn;
     }
   }
 }

@@ -300,15 +300,6 @@ __device__ __forceinline__ void weight_grad_blocks(const RowsArgs &p, const __bf
   }
 }
 
-// a workgroup barrier that orders LDS only: __syncthreads() would also wait
-// for every global load and store in flight (the weights still arriving,
-// the partial gradients still being written)
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // row `row`'s normalised inputs 3c8 .. 3c8+2 and target c8 (zero past n)
 struct Inputs {
   float x[3], t;

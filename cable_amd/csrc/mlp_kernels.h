@@ -93,4 +93,13 @@ __host__ __device__ inline float hswish_grad(float z) {
   return z < -3.0f ? 0.0f : z > 3.0f ? 1.0f : (2.0f * z + 3.0f) * (1.0f / 6.0f);
 }
 
+// a workgroup barrier that orders LDS only: __syncthreads() would also wait
+// for every global load and store in flight (weights or inputs still
+// arriving, results still being written)
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 }  // namespace cbm_mlp
