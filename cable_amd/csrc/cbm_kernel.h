@@ -526,7 +526,9 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
   __shared__ float nrm[NNORM];
   for (int i = threadIdx.x; i < NVEGCLASS * VC_N; i += blockDim.x) tv[i] = a.vtab[i];
   for (int i = threadIdx.x; i < NSOILCLASS * SC_N; i += blockDim.x) ts[i] = a.stab[i];
-  if (threadIdx.x < NNORM) nrm[threadIdx.x] = a.norm[threadIdx.x];
+  // (the inputs' standard deviations as reciprocals)
+  if (threadIdx.x < NNORM)
+    nrm[threadIdx.x] = threadIdx.x >= NIN && threadIdx.x < 2 * NIN ? 1.0f / a.norm[threadIdx.x] : a.norm[threadIdx.x];
   const FeatureSrc src = feature_src(a);
   const int64_t G = gridDim.x;
   const bool feat_lane = threadIdx.x < TILES;
@@ -608,7 +610,7 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
             for (int c = 0; c < NIN; c++) xraw[t * NIN + c] = x[c];
           }
 #pragma unroll
-          for (int c = 0; c < NIN; c++) x[c] = (x[c] - nrm[c]) / nrm[NIN + c];
+          for (int c = 0; c < NIN; c++) x[c] = (x[c] - nrm[c]) * nrm[NIN + c];
         } else {
 #pragma unroll
           for (int c = 0; c < NIN; c++) x[c] = 0.0f;
