@@ -487,13 +487,14 @@ constexpr int OFF1 = 0, OFF2 = OFF1 + HID * P1, OFF3 = OFF2 + HID * PH, OFF4 = O
 constexpr int WLDS = OFF4 + NOUT * PH;           // 41,024 BF16
 constexpr int ALDS = WAVES * 32 * AP;            // 34,816 BF16
 constexpr int BLDS = 3 * HID + NOUT;             // biases, FP32
-static_assert(2 * (WLDS + ALDS) + 4 * BLDS + 8 * (NVEGCLASS * VC_N + NSOILCLASS * SC_N) + 4 * (VCHUNK + NNORM) <=
+static_assert(2 * (WLDS + ALDS) + 4 * BLDS + 8 * (NVEGCLASS * VC_N + NSOILCLASS * SC_N) + 4 * (VCHUNK + NNORM + 3 * HID) <=
                   160 * 1024,
               "fits the 160 KB of LDS");
 // the transposed global layout: per layer [out][in], layers back to back
 constexpr int G1 = 0, G2 = G1 + HID * NIN, G3 = G2 + HID * HID, G4 = G3 + HID * HID, GTOT = G4 + NOUT * HID;
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 }  // namespace infer
 
 // (a template so that the header's kernel has one definition across the
@@ -508,6 +509,7 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
   __shared__ __attribute__((aligned(16))) __bf16 wl[WLDS];
   __shared__ __attribute__((aligned(16))) __bf16 act[ALDS];
   __shared__ __attribute__((aligned(16))) float bl[BLDS];
+  __shared__ uint32_t bpk[3 * HID];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31;
   const int kh = 8 * (lane >> 5), kq = 4 * (lane >> 5);
   __bf16 *A = &act[wave * 32 * AP];
@@ -595,6 +597,13 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
           *(uint2 *)&wl[OFF1 + (u >> 1) * P1 + NIN + 4 * (u & 1)] = make_uint2(0u, 0u);
         for (int u = threadIdx.x; u < BLDS; u += blockDim.x)
           bl[u] = params[u < 3 * HID ? LAYER_B[u / HID] + u % HID : LAYER_B[3] + u - 3 * HID];
+        // the hidden layers' biases as BF16 pairs: hi = b rounded, lo = the
+        // rest rounded (b - hi is exact in FP32), so hi + lo keeps 16 bits
+        for (int u = threadIdx.x; u < 3 * HID; u += blockDim.x) {
+          const float b = params[LAYER_B[u / HID] + u % HID];
+          const uint16_t hi = f32_to_bf16(b), lo = f32_to_bf16(b - mlp::bf16_to_f32(hi));
+          bpk[u] = (uint32_t)hi | ((uint32_t)lo << 16);
+        }
         staged = true;
       }
       // features of tile threadIdx.x, normalised, as BF16 row threadIdx.x
@@ -638,26 +647,34 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
         const __bf16 *W = &wl[l == 0 ? OFF1 : (l == 1 ? OFF2 : OFF3)];
         const int pw = l == 0 ? P1 : PH;
         // lane: tile `col`; register group g of block nb: outputs nb*32 + 8g + kq + 0..3.
-        // The accumulators start at the bias, so the products add onto it.
-        const float *b = &bl[l * HID];
+        // The accumulators start at the bias, formed by one more product:
+        // each output's BF16 pair (hi, lo) in inputs 0 and 1 of its row,
+        // against inputs fixed at one (a lane of the upper half holds inputs
+        // 8..15, all zero).  Read per output row, the bias costs one 4-byte
+        // LDS read per block of 32 outputs instead of sixteen 16-byte ones.
         v16f acc[4];
-#pragma unroll
-        for (int nb = 0; nb < 4; nb++)
-#pragma unroll
-          for (int g = 0; g < 4; g++) {
-            const float4 bv = *(const float4 *)&b[nb * 32 + 8 * g + kq];
-            acc[nb][4 * g + 0] = bv.x;
-            acc[nb][4 * g + 1] = bv.y;
-            acc[nb][4 * g + 2] = bv.z;
-            acc[nb][4 * g + 3] = bv.w;
-          }
-        for (int k0 = 0; k0 < K; k0 += 16) {
-          const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
+        {
+          const v4u ones = {kh == 0 ? 0x3F803F80u : 0u, 0u, 0u, 0u};
+          const v16f zero = {};
 #pragma unroll
           for (int nb = 0; nb < 4; nb++) {
-            const bf8 wv = *(const bf8 *)&W[(nb * 32 + col) * pw + k0 + kh];
-            acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xv, acc[nb], 0, 0, 0);
+            const v4u bw = {kh == 0 ? bpk[l * HID + nb * 32 + col] : 0u, 0u, 0u, 0u};
+            acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, bw),
+                                                              __builtin_bit_cast(bf8, ones), zero, 0, 0, 0);
           }
+        }
+        for (int k0 = 0; k0 < K; k0 += 16) {
+          // the step's five operand reads issued together, then its four products
+          const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
+          bf8 wv[4];
+#pragma unroll
+          for (int nb = 0; nb < 4; nb++) wv[nb] = *(const bf8 *)&W[(nb * 32 + col) * pw + k0 + kh];
+#pragma unroll
+          for (int nb = 0; nb < 4; nb++) acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv[nb], xv, acc[nb], 0, 0, 0);
+          // keep that order: left to itself the scheduler reads each operand
+          // just before its product and waits out every LDS latency
+          __builtin_amdgcn_sched_group_barrier(0x0100, 5, 0);  // LDS reads
+          __builtin_amdgcn_sched_group_barrier(0x0008, 4, 0);  // matrix products
         }
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -674,18 +691,28 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
       }
       // output layer: 8 outputs, rows 8..31 of the weight operand are zero
       {
-        v16f acc;
+        // every operand read first, then the eight products on two
+        // accumulators (two chains of dependent products instead of one)
+        constexpr int KS = HID / 16;
+        bf8 xo[KS], wo[KS];
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-        for (int k0 = 0; k0 < HID; k0 += 16) {
-          const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
-          bf8 wv;
-          if (col < NOUT) wv = *(const bf8 *)&wl[OFF4 + col * PH + k0 + kh];
+        for (int s = 0; s < KS; s++) {
+          xo[s] = *(const bf8 *)&A[col * AP + 16 * s + kh];
+          if (col < NOUT) wo[s] = *(const bf8 *)&wl[OFF4 + col * PH + 16 * s + kh];
           else
 #pragma unroll
-            for (int e = 0; e < 8; e++) wv[e] = (__bf16)0.0f;
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv, xv, acc, 0, 0, 0);
+            for (int e = 0; e < 8; e++) wo[s][e] = (__bf16)0.0f;
         }
+        v16f acc = {}, acc1 = {};
+#pragma unroll
+        for (int s = 0; s < KS; s += 2) {
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wo[s], xo[s], acc, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wo[s + 1], xo[s + 1], acc1, 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x0100, 2 * KS, 0);  // LDS reads
+        __builtin_amdgcn_sched_group_barrier(0x0008, KS, 0);      // matrix products
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q] += acc1[q];
         // lane: tile `col`, outputs kq + 0..3 in registers 0..3
         const int64_t t = t0 + wave * 32 + col;
         if (t < a.mp) {
