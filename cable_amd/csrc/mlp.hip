@@ -194,16 +194,23 @@ __device__ __forceinline__ v16f zero16() {
 // D = W^T X^T for this wave's 32 output features over the block's 32 rows;
 // KS steps of K 16.  Two accumulators halve the dependent chain.
 template <int KS>
-__device__ __forceinline__ v16f product(const bf8 *W, const __bf16 *in, int col, int kh) {
-  v16f a0 = zero16(), a1 = zero16();
+__device__ __forceinline__ v16f product(const bf8 *W, const __bf16 *in, int col, int kh, v16f a0 = zero16()) {
+  v16f a1 = zero16();
+  // every operand read issued before the first product: left to itself the
+  // scheduler reads each one just before its product and, at one wave per
+  // SIMD, waits out every LDS latency
+  bf8 xv[KS];
+#pragma unroll
+  for (int s = 0; s < KS; s++) xv[s] = *(const bf8 *)&in[col * AP + 16 * s + kh];
 #pragma unroll
   for (int s = 0; s < KS; s++) {
-    const bf8 xv = *(const bf8 *)&in[col * AP + 16 * s + kh];
     if (s & 1)
-      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[s], xv, a1, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[s], xv[s], a1, 0, 0, 0);
     else
-      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[s], xv, a0, 0, 0, 0);
+      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[s], xv[s], a0, 0, 0, 0);
   }
+  __builtin_amdgcn_sched_group_barrier(0x0100, KS, 0);  // LDS reads
+  __builtin_amdgcn_sched_group_barrier(0x0008, KS, 0);  // matrix products
   if (KS > 1) a0 += a1;
   return a0;
 }
@@ -214,14 +221,24 @@ __device__ __forceinline__ v16f product(const bf8 *W, const __bf16 *in, int col,
 template <int KS>
 __device__ __forceinline__ void forward_layer(const bf8 *W, const __bf16 *in, __bf16 *out, const float *b,
                                               float *sg, __bf16 *ht, int wave, int col, int kh, int kq) {
-  const v16f acc = product<KS>(W, in, col, kh);
+  // the products add onto the bias
+  v16f b16;
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const float4 bv = *(const float4 *)&b[32 * wave + 8 * g + kq];
+    b16[4 * g + 0] = bv.x;
+    b16[4 * g + 1] = bv.y;
+    b16[4 * g + 2] = bv.z;
+    b16[4 * g + 3] = bv.w;
+  }
+  const v16f acc = product<KS>(W, in, col, kh, b16);
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const int j = 32 * wave + 8 * g + kq;
     bf4 hv;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const float z = acc[4 * g + q] + b[j + q];
+      const float z = acc[4 * g + q];
       sg[4 * g + q] = hswish_grad(z);
       hv[q] = (__bf16)hswish(z);
     }
@@ -263,7 +280,7 @@ struct RowsLDS {
   __attribute__((aligned(16))) __bf16 act[2][RW * AP];
   __attribute__((aligned(16))) __bf16 Ht[HT_ROWS * TP];
   __attribute__((aligned(16))) __bf16 Dt[DT_ROWS * TP];
-  float bias[NBIAS];
+  __attribute__((aligned(16))) float bias[NBIAS];
   float tgt[RW * NOUT];
 };
 
