@@ -99,6 +99,26 @@ std::mutex g_scratch_mu;
 __global__ void zero_counters(unsigned long long *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
+
+// which of nst (at most 64) consecutive steps of the resident forcing window,
+// the first at a.foff, have a leaf-bearing tile in daylight: bit s of *mask
+// for step s.  The test is the inference's own (cbm::mlp_infer_kernel).
+constexpr int LIT_CHUNK = 64;
+__global__ void __launch_bounds__(256) lit_steps_kernel(cbm::StepArgs a, int nst, unsigned long long *mask) {
+  const int *iveg = (const int *)a.F[CABLE_F_veg_iveg];
+  const float *fsd = (const float *)a.F[CABLE_F_met_fsd] + a.foff;
+  unsigned long long m = 0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.mp; t += (int64_t)gridDim.x * blockDim.x) {
+    if (iveg[t] >= 14) continue;
+    for (int s = 0; s < nst; s++) {
+      const float f = fsd[(int64_t)s * a.mp + t];
+      if (f + f > 0.001f) m |= 1ull << s;
+    }
+  }
+  // an OR over the wave, one atomic per wave
+  for (int k = 32; k > 0; k >>= 1) m |= __shfl_xor(m, k, 64);
+  if ((threadIdx.x & 63) == 0 && m) atomicOr(mask, m);
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -141,6 +161,7 @@ struct cable_gpu_ctx {
   float *sparams = nullptr, *snorm = nullptr, *pred = nullptr;  // network, [mp][8]
   uint16_t *swt = nullptr;          // transposed BF16 weights, per layer [out][in]
   unsigned long long *nfallback = nullptr;  // tiles that fell back, summed over a run
+  unsigned long long *d_lit = nullptr;      // the hybrid run's daylight steps (bit mask)
   int64_t last_fallbacks = 0, last_tile_steps = 0;
   // ---- the driver's consistency check ----
   bool cons_on = false;
@@ -460,7 +481,7 @@ int cable_gpu_destroy(cable_gpu_ctx *c) {
   if (c->d_stab) hipFree(c->d_stab);
   if (c->d_npass) hipFree(c->d_npass);
   void *sur[] = {c->cons_part, c->cons_acc, c->hyb, c->sx, c->sy, c->scount, c->sparams, c->snorm, c->pred, c->swt,
-                 c->nfallback};
+                 c->nfallback, c->d_lit};
   for (void *p : sur)
     if (p) hipFree(p);
   if (c->h_stage) hipHostFree(c->h_stage);
@@ -1411,6 +1432,7 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
     HIPCHECK(hipMemset(c->pred, 0, sizeof(float) * cbm_mlp::NOUT * (size_t)c->mp));
   }
   if (!c->nfallback) HIPCHECK(hipMalloc(&c->nfallback, sizeof(unsigned long long)));
+  if (!c->d_lit) HIPCHECK(hipMalloc(&c->d_lit, sizeof(unsigned long long)));
   while (c->kev.size() < 4 * (size_t)nsteps) {
     hipEvent_t ev;
     HIPCHECK(hipEventCreate(&ev));
@@ -1419,8 +1441,29 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
   HIPCHECK(hipMemsetAsync(c->nfallback, 0, sizeof(unsigned long long), c->stream));
   HIPCHECK(hipMemsetAsync(c->d_npass, 0, sizeof(unsigned long long), c->stream));
   HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  unsigned long long lit = 0;
   for (int k = k0; k < k0 + nsteps; k++) {
     StepArgs a = make_args(c, c->d_tab_run, k + 1, dels, (int64_t)(k - c->forc_k0) * c->mp);
+    if ((k - k0) % LIT_CHUNK == 0) {
+      // the steps of the next chunk with a leaf-bearing tile in daylight (one
+      // scan of their shortwave and one wait per 64 steps)
+      const int nst = std::min(LIT_CHUNK, k0 + nsteps - k);
+      hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, c->stream, c->d_lit, 1);
+      const int64_t blocks = std::min<int64_t>((c->mp + 255) / 256, 2048);
+      hipLaunchKernelGGL(lit_steps_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, a, nst, c->d_lit);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(&lit, c->d_lit, sizeof lit, hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipStreamSynchronize(c->stream));
+    }
+    // a step with no leaf-bearing tile in daylight warm-starts nothing: it
+    // runs no network and the plain step (the same bits as the hybrid step
+    // there).  Before a daylight step, or where the next step is not known
+    // (the chunk's or the run's last), that step also keeps the leaf state
+    // the next step's features read.
+    const int s = (k - k0) % LIT_CHUNK;
+    const bool day = (lit >> s) & 1ull;
+    const bool next_day = s + 1 == LIT_CHUNK || k + 1 == k0 + nsteps || ((lit >> (s + 1)) & 1ull);
+    const int mode = day ? MODE_HYBRID : next_day ? MODE_UNLIT : MODE_PLAIN;
     a.hyb = c->hyb;
     a.hyb_valid = c->hyb_valid ? 1 : 0;
     a.norm = c->snorm;
@@ -1431,12 +1474,12 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
     hipError_t e = hipSuccess;
     hipEvent_t *ev = &c->kev[4 * (size_t)(k - k0)];
     HIPCHECK(hipEventRecord(ev[0], c->stream));
-    if (use_surrogate) e = launch_mlp_infer(a, c->swt, c->sparams, c->pred, nullptr, c->mp, c->stream);
+    if (use_surrogate && day) e = launch_mlp_infer(a, c->swt, c->sparams, c->pred, nullptr, c->mp, c->stream);
     HIPCHECK(hipEventRecord(ev[1], c->stream));
     // the capped step; a tile the residual test rejects falls back to the
     // reference's iteration within the same launch
     HIPCHECK(hipEventRecord(ev[2], c->stream));
-    if (e == hipSuccess) e = launch_mixed_mode(MODE_HYBRID, a, c->mp, c->stream);
+    if (e == hipSuccess) e = launch_mixed_mode(mode, a, c->mp, c->stream);
     HIPCHECK(hipEventRecord(ev[3], c->stream));
     if (e != hipSuccess) return fail(CABLE_E_HIP, "cable_gpu_run_hybrid launch: %s", hipGetErrorString(e));
     c->first_call = false;

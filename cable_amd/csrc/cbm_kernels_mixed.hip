@@ -17,6 +17,7 @@ hipError_t launch_mixed_mode(int mode, const StepArgs &a, int64_t mp, hipStream_
   switch (mode) {
     case MODE_CAPTURE: return launch_cbm<float, false, true, MODE_CAPTURE>(a, mp, stream);
     case MODE_HYBRID: return launch_cbm<float, false, true, MODE_HYBRID>(a, mp, stream);
+    case MODE_UNLIT: return launch_cbm<float, false, true, MODE_UNLIT>(a, mp, stream);
     default: return launch_cbm<float, false, true, MODE_PLAIN>(a, mp, stream);
   }
 }

@@ -117,9 +117,12 @@ struct Tile {
 enum StepMode {
   MODE_PLAIN = 0,    // the reference's method
   MODE_CAPTURE = 1,  // the reference's method, and write surrogate training samples
-  MODE_HYBRID = 2    // surrogate warm start, capped polish passes checked by
+  MODE_HYBRID = 2,   // surrogate warm start, capped polish passes checked by
                      // the leaf energy residual, the reference's iteration
                      // where the check rejects them
+  MODE_UNLIT = 3     // a hybrid step in which no leaf-bearing tile is in
+                     // daylight: the reference's method, and the leaf state
+                     // the next step's surrogate features read
 };
 
 // globals of one step

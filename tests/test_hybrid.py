@@ -90,6 +90,38 @@ def test_night_steps_run_the_reference_iteration():
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+def test_unlit_steps_keep_the_leaf_state_for_the_morning():
+    """A hybrid step with no leaf-bearing tile in daylight runs no network and
+    the plain step, which must still keep the leaf state the next step's
+    features read.  A hybrid run from midnight into the morning (the first
+    daylight step is 06:30) against the capture step, which keeps that state
+    too, through the night and the hybrid from 06:30: with a network and a
+    cap of one pass the morning's results depend on the features, and they
+    give the same bits and the same fallbacks."""
+    from cable_amd.surrogate import NIN, NOUT
+    from oracle import mlp_oracle as O
+    print("[hybrid] unlit steps into the morning", flush=True)
+    wl = Workload(8192, 16)
+    p = O.init_params(7).astype(np.float32)
+    unit = np.concatenate([np.zeros(NIN), np.ones(NIN), np.zeros(NOUT), np.ones(NOUT)]).astype(np.float32)
+    g = _ctx(wl)
+    g.set_surrogate(p, unit)
+    g.run_hybrid(0, 16, wl.dels, max_passes=1, resid_max=0.5, use_surrogate=True)
+    ca = g.hybrid_counters()
+    a = _outputs(g)
+    g.close()
+    g = _ctx(wl)
+    g.set_surrogate(p, unit)
+    g.run_capture(0, 13, wl.dels, 8192 * 13)
+    g.run_hybrid(13, 3, wl.dels, max_passes=1, resid_max=0.5, use_surrogate=True)
+    cb = g.hybrid_counters()
+    b = _outputs(g)
+    g.close()
+    assert ca[0] == cb[0] and ca[0] > 0, (ca, cb)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_hybrid_is_deterministic():
     """The fallback count is summed by atomics in whatever order the waves
     reach it; no tile's result depends on that order.  Two runs of the same
