@@ -31,7 +31,8 @@ EXPORTS = [
     "cable_gpu_step", "cable_gpu_cbm",
     "cable_gpu_veg_cols", "cable_gpu_soil_cols", "cable_gpu_table_column",
     "cable_gpu_set_class_tables", "cable_gpu_set_forcing", "cable_gpu_run",
-    "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_last_kernel_ms", "cable_gpu_counters",
+    "cable_gpu_sync", "cable_gpu_last_run_ms", "cable_gpu_last_kernel_ms", "cable_gpu_set_launch_timing",
+    "cable_gpu_counters",
     "cable_gpu_set_tile_order",
     "cable_gpu_run_capture", "cable_gpu_samples", "cable_gpu_copy_samples", "cable_gpu_set_surrogate",
     "cable_gpu_run_hybrid", "cable_gpu_run_hybrid_noalloc", "cable_gpu_hybrid_counters", "cable_gpu_synth_forcing",
@@ -90,6 +91,7 @@ def lib():
         "cable_gpu_sync": ([vp], c_int),
         "cable_gpu_last_run_ms": ([vp, ctypes.POINTER(c_double)], c_int),
         "cable_gpu_last_kernel_ms": ([vp, ctypes.POINTER(c_double), ctypes.POINTER(c_double)], c_int),
+        "cable_gpu_set_launch_timing": ([vp, c_int], c_int),
         "cable_gpu_counters": ([vp, ctypes.POINTER(i64)], c_int),
         "cable_gpu_last_error": ([], ctypes.c_char_p),
         "cable_gpu_set_tile_order": ([vp, vp], c_int),
@@ -393,6 +395,11 @@ class CableGPU:
         a, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().cable_gpu_last_kernel_ms(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_launch_timing(self, on):
+        """Per-launch events in hybrid runs (on by default; they cost the run
+        time, see include/cable_gpu.h)."""
+        _check(lib().cable_gpu_set_launch_timing(self.h, 1 if on else 0))
 
     def sync(self):
         _check(lib().cable_gpu_sync(self.h))

@@ -255,6 +255,12 @@ int cable_gpu_last_run_ms(cable_gpu_ctx *ctx, double *ms);
    a hybrid run, around the run otherwise.  (No counterpart in the
    reference; a measurement.) */
 int cable_gpu_last_kernel_ms(cable_gpu_ctx *ctx, double *step_ms, double *infer_ms);
+/* Whether a hybrid run records those per-launch events (on = 1, the
+   default).  Each event is a timestamp packet the stream waits on: four per
+   step measured 4% of the hybrid's time per step at 1,048,576 tiles.  Off,
+   cable_gpu_last_kernel_ms gives the whole run as step time and no inference
+   time.  (No counterpart in the reference; a measurement.) */
+int cable_gpu_set_launch_timing(cable_gpu_ctx *ctx, int on);
 
 /* Total dryLeaf fixed-point passes of the last step or run (summed over tiles
    and the four stability iterations), a work counter. */

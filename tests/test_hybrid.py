@@ -122,6 +122,31 @@ def test_unlit_steps_keep_the_leaf_state_for_the_morning():
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+def test_launch_timing_changes_only_the_timing():
+    """Without the per-launch events a hybrid run gives the same bits and
+    counters, and reports the whole run as step time."""
+    from cable_amd.surrogate import NIN, NOUT
+    from oracle import mlp_oracle as O
+    print("[hybrid] launch timing", flush=True)
+    wl = Workload(8192, 16)
+    p = O.init_params(3).astype(np.float32)
+    unit = np.concatenate([np.zeros(NIN), np.ones(NIN), np.zeros(NOUT), np.ones(NOUT)]).astype(np.float32)
+    runs = []
+    for timing in (True, False):
+        g = _ctx(wl)
+        g.set_surrogate(p, unit)
+        g.set_launch_timing(timing)
+        g.run_hybrid(12, 4, wl.dels, max_passes=2, resid_max=0.5, use_surrogate=True)
+        runs.append((g.hybrid_counters(), _outputs(g), g.last_kernel_ms(), g.last_run_ms()))
+        g.close()
+    (c0, a, (s0, i0), _), (c1, b, (s1, i1), r1) = runs
+    assert c0 == c1
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    assert s0 > 0 and i0 > 0
+    assert i1 == 0 and s1 == pytest.approx(r1)
+
+
 def test_hybrid_is_deterministic():
     """The fallback count is summed by atomics in whatever order the waves
     reach it; no tile's result depends on that order.  Two runs of the same
