@@ -303,8 +303,12 @@ int cable_gpu_set_surrogate(cable_gpu_ctx *ctx, const float *params, const float
    up to its 20 passes (src/science/canopy/cbl_dryLeaf.F90:239-606).  Every
    other tile runs the reference's iteration.  use_surrogate = 0 runs the
    same capped method from the reference's own starting point (the
-   comparison without the network).  Two launches per step: the inference
-   and the step. */
+   comparison without the network).  Which steps have a leaf-bearing tile
+   in daylight is found by one scan of the forcing window's shortwave, and
+   one wait, per 64 steps: such a step is two launches, the inference and
+   the step; any other step is one launch of the plain step, which before a
+   daylight step (and at the end of the run or of the 64) also keeps the
+   leaf state the network's inputs read. */
 int cable_gpu_run_hybrid(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, int max_passes,
                          double resid_max, int use_surrogate);
 /* cable_gpu_run_hybrid that allocates nothing.  No counterpart in the
