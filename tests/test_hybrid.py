@@ -122,6 +122,40 @@ def test_unlit_steps_keep_the_leaf_state_for_the_morning():
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+def test_daylight_dispatch_across_days_and_chunks():
+    """A hybrid run of 70 steps in one call (from 22:30: night, a day, night
+    and the next morning), whose daylight scan covers 64 steps at a time,
+    against the same steps in 70 calls of one step each (every call's step
+    keeps the leaf state): the same bits and the same fallbacks, so no step
+    that a later daylight step reads drops the state.  The scan's 64th step
+    is the last night step before a dawn (06:00), so the case where the next
+    step lies beyond the scan is covered too."""
+    from cable_amd.surrogate import NIN, NOUT
+    from oracle import mlp_oracle as O
+    print("[hybrid] daylight dispatch over 70 steps", flush=True)
+    k0 = 45                                   # 22:30; step k0 + 63 is 06:00
+    wl = Workload(3000, k0 + 70)
+    p = O.init_params(13).astype(np.float32)
+    unit = np.concatenate([np.zeros(NIN), np.ones(NIN), np.zeros(NOUT), np.ones(NOUT)]).astype(np.float32)
+    g = _ctx(wl)
+    g.set_surrogate(p, unit)
+    g.run_hybrid(k0, 70, wl.dels, max_passes=1, resid_max=0.5, use_surrogate=True)
+    ca = g.hybrid_counters()
+    a = _outputs(g)
+    g.close()
+    g = _ctx(wl)
+    g.set_surrogate(p, unit)
+    fb = 0
+    for k in range(k0, k0 + 70):
+        g.run_hybrid(k, 1, wl.dels, max_passes=1, resid_max=0.5, use_surrogate=True)
+        fb += g.hybrid_counters()[0]
+    b = _outputs(g)
+    g.close()
+    assert ca[0] == fb and fb > 0, (ca, fb)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_launch_timing_changes_only_the_timing():
     """Without the per-launch events a hybrid run gives the same bits and
     counters, and reports the whole run as step time."""
