@@ -6,7 +6,8 @@
 // layout.  MODE (StepMode) adds the surrogate's work: MODE_CAPTURE records
 // the converged leaf state as a training target; MODE_HYBRID starts the leaf
 // fixed point from the surrogate's prediction, caps its passes and reports
-// whether the tile must fall back to the plain method.
+// whether the tile must fall back to the plain method; MODE_UNLIT is the plain
+// method that also keeps the leaf state the next step's surrogate inputs read.
 #pragma once
 #include "cbm_physics.h"
 
