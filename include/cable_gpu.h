@@ -325,7 +325,10 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, in
    call on a GPU shares that GPU's scratch: calls that overlap on one GPU
    race.  Results, counters (cable_gpu_hybrid_counters, cable_gpu_counters)
    and cable_gpu_last_run_ms as cable_gpu_run_hybrid; no per-kernel split
-   (cable_gpu_last_kernel_ms gives the whole run as step time). */
+   (cable_gpu_last_kernel_ms gives the whole run as step time).  It makes no
+   host wait during the run, so it does not look for the steps without
+   daylight: every step runs the inference and the hybrid step, which give
+   the plain step's bits there, at the cost of the night steps' extra time. */
 int cable_gpu_run_hybrid_noalloc(cable_gpu_ctx *ctx, int k0, int nsteps, double dels, int max_passes,
                                  double resid_max, int use_surrogate);
 /* tile-steps that fell back, and all tile-steps, of the last hybrid run */
