@@ -302,7 +302,7 @@ def profiled(name):
             for line in fh:
                 k, sep, v = line.partition(":")
                 if sep and k.strip() in ("traffic_bytes_per_launch", "valu_issue_utilisation", "fp32_valu_frac",
-                                         "fp64_valu_frac", "mfma_busy_frac"):
+                                         "fp64_valu_frac", "mfma_busy_frac", "mfma_bf16_frac"):
                     try:
                         out[k.strip()] = float(v)
                     except ValueError:
@@ -573,6 +573,9 @@ def main():
                                 "step kernel of the headline form (per-launch HIP events on the context stream)"),
                      "valu": {k: prof[k] for k in ("valu_issue_utilisation", "fp32_valu_frac", "fp64_valu_frac")
                               if k in prof},
+                     "mfma": ({"kernel": "the surrogate's inference (mlp_infer_kernel), daylight launches",
+                               "busy_frac": prof["mfma_busy_frac"], "bf16_frac": prof["mfma_bf16_frac"]}
+                              if "mfma_bf16_frac" in prof else None),
                      "profile": prof.get("source")},
         "cpu_baseline": cpu,
         "forms": forms,
