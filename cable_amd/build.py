@@ -36,7 +36,7 @@ TU_FLAGS = {
     # drops the range handling of the hardware exp/log sequences (3% of the
     # step's instructions; 2.7% faster at 1,048,576 tiles, same pass count)
     "cbm_kernels_mixed.hip": ["-ffp-contract=on", "-fapprox-func", "-fgpu-flush-denormals-to-zero",
-                              "-fno-honor-nans", "-fno-signed-zeros",
+                              "-fno-honor-nans", "-fno-signed-zeros", "-fno-honor-infinities",
                               "-Xarch_device", "-mllvm=-misched=gcn-max-ilp"],
     "mlp.hip": ["-ffp-contract=on"],
 }
