@@ -497,6 +497,62 @@ typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 }  // namespace infer
 
+// One hidden layer of the inference for a wave's 32 tiles, in place in
+// their activation rows A (pitch AP): Z = W^T A^T + b on
+// v_mfma_f32_32x32x16_bf16, then hard-swish, as BF16, back into A.  Lane:
+// tile `col`; register group g of output block nb: outputs nb*32 + 8g + kq +
+// 0..3.  Software-pipelined over the four blocks of 32 outputs: the inputs
+// are read into registers first (the rows are then free to be overwritten),
+// block nb+1's weights are read while block nb's products run, and block
+// nb's activations (vector work) are formed while block nb+1's products run
+// on the matrix cores.  (A second weight buffer spilled registers.)  A block's accumulators start at its bias, formed by
+// one more product: each output's BF16 pair (hi, lo) in inputs 0 and 1 of
+// its row, against inputs fixed at one (a lane of the upper half holds
+// inputs 8..15, all zero).
+template <int KS>
+__device__ __forceinline__ void infer_hidden(const __bf16 *W, int pw, __bf16 *A, const uint32_t *bpk, int col,
+                                             int kh, int kq) {
+  using namespace infer;
+  bf8 xv[KS], w[KS];
+#pragma unroll
+  for (int s = 0; s < KS; s++) xv[s] = *(const bf8 *)&A[col * AP + 16 * s + kh];
+  auto weights = [&](int nb, bf8 *w) {
+#pragma unroll
+    for (int s = 0; s < KS; s++) w[s] = *(const bf8 *)&W[(nb * 32 + col) * pw + 16 * s + kh];
+  };
+  const v4u ones = {kh == 0 ? 0x3F803F80u : 0u, 0u, 0u, 0u};
+  const v16f zero = {};
+  auto products = [&](int nb, const bf8 *w) {
+    const v4u bw = {kh == 0 ? bpk[nb * 32 + col] : 0u, 0u, 0u, 0u};
+    v16f acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, bw), __builtin_bit_cast(bf8, ones),
+                                                       zero, 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < KS; s++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[s], xv[s], acc, 0, 0, 0);
+    return acc;
+  };
+  auto activations = [&](int nb, const v16f &acc) {
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      bf4 v;
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] = (__bf16)cbm_mlp::hswish(acc[4 * g + q]);
+      *(bf4 *)&A[col * AP + nb * 32 + 8 * g + kq] = v;
+    }
+  };
+  weights(0, w);
+  v16f a0 = products(0, w);
+  weights(1, w);
+  v16f a1 = products(1, w);
+  activations(0, a0);
+  weights(2, w);
+  a0 = products(2, w);
+  activations(1, a1);
+  weights(3, w);
+  a1 = products(3, w);
+  activations(2, a0);
+  activations(3, a1);
+}
+
 // (a template so that the header's kernel has one definition across the
 // translation units that include it)
 template <int NWAVES = infer::WAVES>
@@ -641,54 +697,9 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
       }
       // from here on a wave touches only its own 32 activation rows: its LDS
       // operations complete in order, so the layers need no workgroup barrier
+      infer_hidden<K1 / 16>(&wl[OFF1], P1, A, &bpk[0], col, kh, kq);
 #pragma unroll 1
-      for (int l = 0; l < 3; l++) {
-        const int K = l == 0 ? K1 : HID;
-        const __bf16 *W = &wl[l == 0 ? OFF1 : (l == 1 ? OFF2 : OFF3)];
-        const int pw = l == 0 ? P1 : PH;
-        // lane: tile `col`; register group g of block nb: outputs nb*32 + 8g + kq + 0..3.
-        // The accumulators start at the bias, formed by one more product:
-        // each output's BF16 pair (hi, lo) in inputs 0 and 1 of its row,
-        // against inputs fixed at one (a lane of the upper half holds inputs
-        // 8..15, all zero).  Read per output row, the bias costs one 4-byte
-        // LDS read per block of 32 outputs instead of sixteen 16-byte ones.
-        v16f acc[4];
-        {
-          const v4u ones = {kh == 0 ? 0x3F803F80u : 0u, 0u, 0u, 0u};
-          const v16f zero = {};
-#pragma unroll
-          for (int nb = 0; nb < 4; nb++) {
-            const v4u bw = {kh == 0 ? bpk[l * HID + nb * 32 + col] : 0u, 0u, 0u, 0u};
-            acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, bw),
-                                                              __builtin_bit_cast(bf8, ones), zero, 0, 0, 0);
-          }
-        }
-        for (int k0 = 0; k0 < K; k0 += 16) {
-          // the step's five operand reads issued together, then its four products
-          const bf8 xv = *(const bf8 *)&A[col * AP + k0 + kh];
-          bf8 wv[4];
-#pragma unroll
-          for (int nb = 0; nb < 4; nb++) wv[nb] = *(const bf8 *)&W[(nb * 32 + col) * pw + k0 + kh];
-#pragma unroll
-          for (int nb = 0; nb < 4; nb++) acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv[nb], xv, acc[nb], 0, 0, 0);
-          // keep that order: left to itself the scheduler reads each operand
-          // just before its product and waits out every LDS latency
-          __builtin_amdgcn_sched_group_barrier(0x0100, 5, 0);  // LDS reads
-          __builtin_amdgcn_sched_group_barrier(0x0008, 4, 0);  // matrix products
-        }
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int nb = 0; nb < 4; nb++)
-#pragma unroll
-          for (int g = 0; g < 4; g++) {
-            const int j = nb * 32 + 8 * g + kq;
-            bf4 v;
-#pragma unroll
-            for (int q = 0; q < 4; q++) v[q] = (__bf16)hswish(acc[nb][4 * g + q]);
-            *(bf4 *)&A[col * AP + j] = v;
-          }
-        __builtin_amdgcn_wave_barrier();
-      }
+      for (int l = 1; l < 3; l++) infer_hidden<HID / 16>(&wl[l == 1 ? OFF2 : OFF3], PH, A, &bpk[l * HID], col, kh, kq);
       // output layer: 8 outputs, rows 8..31 of the weight operand are zero
       {
         // every operand read first, then the eight products on two
