@@ -333,6 +333,9 @@ CBM_HD void step_tile(const StepArgs &a, int64_t t, const double *vtab, const do
       }
     }
   };
+  // the previous-step canopy state the surrogate's features read (written
+  // by the step at the end of the canopy)
+  if constexpr (MODE != MODE_PLAIN) S.hyb_out = a.hyb ? a.hyb + t : nullptr;
   S.run(late, tail);
   if (a.driver) S.sumcflux();
   o.passes = S.npass;
@@ -347,14 +350,6 @@ CBM_HD void step_tile(const StepArgs &a, int64_t t, const double *vtab, const do
     o.emit = S.vegetated;
 #pragma unroll
     for (int c = 0; c < mlp::NOUT; c++) o.tgt[c] = S.tgt[c];
-  }
-  // the previous-step canopy state the surrogate's features read
-  if constexpr (MODE != MODE_PLAIN) {
-    if (a.hyb) {
-      a.hyb[t] = (float)S.leaf_temperature;
-      a.hyb[mp + t] = (float)T.met_tvair[0];
-      a.hyb[2 * mp + t] = (float)T.met_qvair[0];
-    }
   }
   // ---- scatter ----
   if constexpr (FULL) {

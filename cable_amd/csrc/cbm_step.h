@@ -28,7 +28,8 @@ struct Stepper {
   bool vegetated = false;
   bool capped = false;         // MODE_HYBRID: a sunlit leaf-bearing tile, whose
                                // leaf passes are capped and checked
-  R leaf_temperature = R(0);   // the leaf fixed point's final temperature
+  float *hyb_out = nullptr;     // MODE != MODE_PLAIN: the tile's leaf temperature,
+                               // canopy-air temperature and humidity, `a.mp` apart
 
   // ---- constants: src/params/*, src/util/cable_common.F90:217-222 ----
   static constexpr R TFRZ = R(273.16), SBOLTZ = R(5.67e-8), EMSOIL = R(1.0),
@@ -1114,7 +1115,16 @@ struct Stepper {
       tgt[6] = (float)c.ecy;
       tgt[7] = (float)c.hcy;
     }
-    if constexpr (MODE != MODE_PLAIN) leaf_temperature = c.tlfx;
+    if constexpr (MODE != MODE_PLAIN) {
+      // the previous-step canopy state the next step's surrogate inputs
+      // read, written here where it is final rather than held through soil
+      // and snow
+      if (hyb_out) {
+        hyb_out[0] = (float)c.tlfx;
+        hyb_out[a.mp] = (float)t.met_tvair[0];
+        hyb_out[2 * a.mp] = (float)t.met_qvair[0];
+      }
+    }
     R us = t.canopy_us[0], ua = t.met_ua[0];
     R rho = t.air_rho[0];
     R rlam = t.air_rlam[0];
