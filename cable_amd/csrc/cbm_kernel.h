@@ -606,13 +606,37 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
       }
       return VCHUNK;
     };
-    // a block's feature loads are issued before the previous block's network
-    // and finished after it, so their latency hides behind the matrix work
+    // A block's features are loaded before the previous block's network and
+    // finished (the arithmetic, the normalisation, BF16) in its middle, by
+    // the threads of waves 0-3 while the waves sharing their SIMDs compute;
+    // they wait in registers for the rows to be free.
     FeatureRaw<float> raw;
+    bf8 fr[3];
+    auto finish = [&](int64_t t) {
+      float x[NIN];
+      if (t < a.mp) {
+        const int iv = raw.iveg - 1, is = raw.isoilm - 1;
+        finish_features<float>(a, raw, &tv[(iv < 0 ? 0 : iv) * VC_N], &ts[(is < 0 ? 0 : is) * SC_N], x);
+        if (xraw) {
+#pragma unroll
+          for (int c = 0; c < NIN; c++) xraw[t * NIN + c] = x[c];
+        }
+#pragma unroll
+        for (int c = 0; c < NIN; c++) x[c] = (x[c] - nrm[c]) * nrm[NIN + c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < NIN; c++) x[c] = 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) fr[j][e] = (__bf16)x[8 * j + e];
+    };
     int qb = next_block(0);
     if (qb < VCHUNK && feat_lane) {
       const int64_t t = (c0 + (int64_t)qb * G) * TILES + threadIdx.x;
       if (t < a.mp) load_feature_raw<float>(a, src, t, raw);
+      finish(t);
     }
     while (qb < VCHUNK) {
       const int64_t t0 = (c0 + (int64_t)qb * G) * TILES;
@@ -657,44 +681,25 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
         }
         staged = true;
       }
-      // features of tile threadIdx.x, normalised, as BF16 row threadIdx.x
+      // features of tile threadIdx.x as BF16 row threadIdx.x (24 inputs and
+      // zeros to the padded K of 32)
       if (feat_lane) {
-        const int64_t t = t0 + threadIdx.x;
         __bf16 *row = &act[threadIdx.x * AP];
-        float x[NIN];
-        if (t < a.mp) {
-          const int iv = raw.iveg - 1, is = raw.isoilm - 1;
-          finish_features<float>(a, raw, &tv[(iv < 0 ? 0 : iv) * VC_N], &ts[(is < 0 ? 0 : is) * SC_N], x);
-          if (xraw) {
 #pragma unroll
-            for (int c = 0; c < NIN; c++) xraw[t * NIN + c] = x[c];
-          }
-#pragma unroll
-          for (int c = 0; c < NIN; c++) x[c] = (x[c] - nrm[c]) * nrm[NIN + c];
-        } else {
-#pragma unroll
-          for (int c = 0; c < NIN; c++) x[c] = 0.0f;
-        }
-#pragma unroll
-        for (int c = 0; c < NIN; c += 4) {
-          bf4 v = {(__bf16)x[c], (__bf16)x[c + 1], (__bf16)x[c + 2], (__bf16)x[c + 3]};
-          *(bf4 *)&row[c] = v;
-        }
-        *(uint2 *)&row[NIN] = make_uint2(0u, 0u);
-        *(uint2 *)&row[NIN + 4] = make_uint2(0u, 0u);
+        for (int j = 0; j < 3; j++) *(bf8 *)&row[8 * j] = fr[j];
+        *(uint4 *)&row[NIN] = make_uint4(0u, 0u, 0u, 0u);
       }
       // (LDS only: the staged weights and the feature rows)
       lds_barrier();
       const int qn = next_block(qb + 1);
-      if (qn < VCHUNK && feat_lane) {
-        const int64_t t = (c0 + (int64_t)qn * G) * TILES + threadIdx.x;
-        if (t < a.mp) load_feature_raw<float>(a, src, t, raw);
-      }
+      const int64_t tn = (c0 + (int64_t)qn * G) * TILES + threadIdx.x;
+      if (qn < VCHUNK && feat_lane && tn < a.mp) load_feature_raw<float>(a, src, tn, raw);
       // from here on a wave touches only its own 32 activation rows: its LDS
       // operations complete in order, so the layers need no workgroup barrier
       infer_hidden<K1 / 16>(&wl[OFF1], P1, A, &bpk[0], col, kh, kq);
-#pragma unroll 1
-      for (int l = 1; l < 3; l++) infer_hidden<HID / 16>(&wl[l == 1 ? OFF2 : OFF3], PH, A, &bpk[l * HID], col, kh, kq);
+      infer_hidden<HID / 16>(&wl[OFF2], PH, A, &bpk[HID], col, kh, kq);
+      if (qn < VCHUNK && feat_lane) finish(tn);
+      infer_hidden<HID / 16>(&wl[OFF3], PH, A, &bpk[2 * HID], col, kh, kq);
       // output layer: 8 outputs, rows 8..31 of the weight operand are zero
       {
         // every operand read first, then the eight products on two
