@@ -102,21 +102,30 @@ __global__ void zero_counters(unsigned long long *p, int n) {
 
 // which of nst (at most 64) consecutive steps of the resident forcing window,
 // the first at a.foff, have a leaf-bearing tile in daylight: bit s of *mask
-// for step s.  The test is the inference's own (cbm::mlp_infer_kernel).
+// for step s, and of blocks[b] for the tiles of block b (256 tiles, the
+// inference's block; zeroed by the caller).  The test is the inference's own
+// (cbm::mlp_infer_kernel).
 constexpr int LIT_CHUNK = 64;
-__global__ void __launch_bounds__(256) lit_steps_kernel(cbm::StepArgs a, int nst, unsigned long long *mask) {
+__global__ void __launch_bounds__(256) lit_steps_kernel(cbm::StepArgs a, int nst, unsigned long long *mask,
+                                                        unsigned long long *blocks) {
   const int *iveg = (const int *)a.F[CABLE_F_veg_iveg];
   const float *fsd = (const float *)a.F[CABLE_F_met_fsd] + a.foff;
   unsigned long long m = 0;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.mp; t += (int64_t)gridDim.x * blockDim.x) {
-    if (iveg[t] >= 14) continue;
-    for (int s = 0; s < nst; s++) {
-      const float f = fsd[(int64_t)s * a.mp + t];
-      if (f + f > 0.001f) m |= 1ull << s;
-    }
+  // whole waves step together, so that a wave's 64 tiles lie in one block
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < a.mp; t0 += stride) {
+    const int64_t t = t0 + (threadIdx.x & 63);
+    unsigned long long mt = 0;
+    if (t < a.mp && iveg[t] < 14)
+      for (int s = 0; s < nst; s++) {
+        const float f = fsd[(int64_t)s * a.mp + t];
+        if (f + f > 0.001f) mt |= 1ull << s;
+      }
+    // an OR over the wave, one atomic per wave and block
+    for (int k = 32; k > 0; k >>= 1) mt |= __shfl_xor(mt, k, 64);
+    if ((threadIdx.x & 63) == 0 && mt) atomicOr(&blocks[t0 / 256], mt);
+    m |= mt;
   }
-  // an OR over the wave, one atomic per wave
-  for (int k = 32; k > 0; k >>= 1) m |= __shfl_xor(m, k, 64);
   if ((threadIdx.x & 63) == 0 && m) atomicOr(mask, m);
 }
 }  // namespace
@@ -163,6 +172,7 @@ struct cable_gpu_ctx {
   uint16_t *swt = nullptr;          // transposed BF16 weights, per layer [out][in]
   unsigned long long *nfallback = nullptr;  // tiles that fell back, summed over a run
   unsigned long long *d_lit = nullptr;      // the hybrid run's daylight steps (bit mask)
+  unsigned long long *d_litblk = nullptr;   // the same per block of 256 tiles
   int64_t last_fallbacks = 0, last_tile_steps = 0;
   // ---- the driver's consistency check ----
   bool cons_on = false;
@@ -482,7 +492,7 @@ int cable_gpu_destroy(cable_gpu_ctx *c) {
   if (c->d_stab) hipFree(c->d_stab);
   if (c->d_npass) hipFree(c->d_npass);
   void *sur[] = {c->cons_part, c->cons_acc, c->hyb, c->sx, c->sy, c->scount, c->sparams, c->snorm, c->pred, c->swt,
-                 c->nfallback, c->d_lit};
+                 c->nfallback, c->d_lit, c->d_litblk};
   for (void *p : sur)
     if (p) hipFree(p);
   if (c->h_stage) hipHostFree(c->h_stage);
@@ -1434,6 +1444,8 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
   }
   if (!c->nfallback) HIPCHECK(hipMalloc(&c->nfallback, sizeof(unsigned long long)));
   if (!c->d_lit) HIPCHECK(hipMalloc(&c->d_lit, sizeof(unsigned long long)));
+  const int64_t nlitblk = (c->mp + 255) / 256;
+  if (!c->d_litblk) HIPCHECK(hipMalloc(&c->d_litblk, sizeof(unsigned long long) * (size_t)nlitblk));
   while (c->kev.size() < 4 * (size_t)nsteps) {
     hipEvent_t ev;
     HIPCHECK(hipEventCreate(&ev));
@@ -1450,8 +1462,10 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
       // scan of their shortwave and one wait per 64 steps)
       const int nst = std::min(LIT_CHUNK, k0 + nsteps - k);
       hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, c->stream, c->d_lit, 1);
+      HIPCHECK(hipMemsetAsync(c->d_litblk, 0, sizeof(unsigned long long) * (size_t)nlitblk, c->stream));
       const int64_t blocks = std::min<int64_t>((c->mp + 255) / 256, 2048);
-      hipLaunchKernelGGL(lit_steps_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, a, nst, c->d_lit);
+      hipLaunchKernelGGL(lit_steps_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, a, nst, c->d_lit,
+                         c->d_litblk);
       HIPCHECK(hipGetLastError());
       HIPCHECK(hipMemcpyAsync(&lit, c->d_lit, sizeof lit, hipMemcpyDeviceToHost, c->stream));
       HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1472,6 +1486,8 @@ int cable_gpu_run_hybrid(cable_gpu_ctx *c, int k0, int nsteps, double dels, int 
     a.max_passes = max_passes;
     a.resid_max = (float)resid_max;
     a.nfallback = c->nfallback;
+    a.lit_blocks = c->d_litblk;
+    a.lit_bit = s;
     hipError_t e = hipSuccess;
     hipEvent_t *ev = &c->kev[4 * (size_t)(k - k0)];
     if (c->launch_timing) HIPCHECK(hipEventRecord(ev[0], c->stream));

@@ -474,6 +474,7 @@ cbm_kernel(StepArgs a) {
 namespace infer {
 using namespace mlp;
 constexpr int WAVES = 8, TILES = 32 * WAVES;
+static_assert(TILES == 256, "the hybrid run's daylight scan votes on blocks of 256 tiles");
 constexpr int VCHUNK = 64;                       // blocks voted on at once
 constexpr int K1 = 32;                           // layer-1 K padded to a multiple of 16
 constexpr int P1 = K1 + 8, PH = HID + 8;         // weight row pitches (BF16)
@@ -587,9 +588,13 @@ __global__ void __launch_bounds__(64 * NWAVES) mlp_infer_kernel(StepArgs a, cons
   const bool feat_lane = threadIdx.x < TILES;
   for (int64_t c0 = blockIdx.x; c0 < nblk; c0 += G * VCHUNK) {
     __syncthreads();  // the previous chunk's blocks are done with vote[]
-    if (threadIdx.x < VCHUNK) vote[threadIdx.x] = 0;
+    if (threadIdx.x < VCHUNK) {
+      // the votes the caller's daylight scan already took, or none yet
+      const int64_t b = c0 + (int64_t)threadIdx.x * G;
+      vote[threadIdx.x] = a.lit_blocks && b < nblk ? (int)((a.lit_blocks[b] >> a.lit_bit) & 1ull) : 0;
+    }
     __syncthreads();
-    for (int i = threadIdx.x; i < VCHUNK * TILES; i += blockDim.x) {
+    for (int i = threadIdx.x; !a.lit_blocks && i < VCHUNK * TILES; i += blockDim.x) {
       const int64_t b = c0 + (int64_t)(i / TILES) * G, t = b * TILES + i % TILES;
       if (b < nblk && t < a.mp && ld<int>(src.iveg, t) < 14) {
         const float fsd = ld<float>(src.fsd, t + a.foff);

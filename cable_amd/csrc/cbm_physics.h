@@ -157,6 +157,10 @@ struct StepArgs {
   int max_passes;            // MODE_HYBRID: dryLeaf passes per NITER pass
   float resid_max;           // MODE_HYBRID: leaf energy residual limit, W/m2
   unsigned long long *nfallback;  // MODE_HYBRID: tiles that fell back (counter)
+  const unsigned long long *lit_blocks;  // optional: per block of 256 tiles, bit
+                             // `lit_bit` set where a leaf-bearing tile is in
+                             // daylight at this step (the inference's vote)
+  int lit_bit;
 };
 
 // literal of the working precision
